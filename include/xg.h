@@ -1,0 +1,304 @@
+/*
+ * xg.h — C ABI of the MI355X (gfx950) evaluator for xtensor's lazy-expression
+ * hot path: broadcast assignment, axis reductions and scans.
+ *
+ * Everything that crosses this boundary is a plain C type: pointers, sizes,
+ * enums and fixed-size structs.  No C++ or torch types appear here.  A front
+ * end (xtensor's expression-tag layer in C++, or the Python package shipped
+ * beside this library) describes an expression tree with `xg_expr`, names the
+ * output with `xg_tensor`, and calls one entry point.  All device buffers
+ * belong to the caller; the library never allocates or frees them.  Calls are
+ * stream-ordered: they enqueue work on `stream` and return without waiting.
+ *
+ * Reference interfaces that each entry point replaces (xtensor 0.27.1):
+ *   xg_assign  <- xexpression_assigner<Tag>::assign_xexpression / assign_data
+ *                 include/xtensor/core/xassign.hpp:208-226, 439-486
+ *                 (reached from xt::noalias(out) = e, core/xnoalias.hpp:160-165,
+ *                  and xview_semantic::assign_xexpression, core/xsemantic.hpp:724-731)
+ *   xg_reduce  <- xt::reduce / xreducer + xreducer_stepper::aggregate
+ *                 include/xtensor/reducers/xreducer.hpp:915-1079, 1404-1462, 1778-1828
+ *                 and the sum/prod/amax/amin/mean entry points,
+ *                 include/xtensor/core/xmath.hpp:367-423, 777-800, 1803-1912,
+ *                 and norm_sq/norm_l2/norm_l1/norm_linf/norm_l0,
+ *                 include/xtensor/reducers/xnorm.hpp:397-526
+ *   xg_scan    <- xt::accumulate / accumulator_impl and cumsum/cumprod,
+ *                 include/xtensor/reducers/xaccumulator.hpp:215-367,
+ *                 include/xtensor/core/xmath.hpp:2247-2301
+ *   xg_expr_info / xg_reduce_info / xg_scan_info
+ *              <- xfunction::broadcast_shape + value_type deduction,
+ *                 include/xtensor/core/xfunction.hpp:133-142, 537-544, 709-727,
+ *                 include/xtensor/core/xstrides.hpp:737-794
+ *   status codes <- broadcast_error (utils/xexception.hpp:63) and the
+ *                 std::runtime_error throws of reducers/xreducer.hpp:1426-1440
+ *                 and reducers/xaccumulator.hpp:231-234.
+ */
+#ifndef XG_H
+#define XG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same type as HIP's hipStream_t (`struct ihipStream_t*`); NULL = null stream. */
+typedef struct ihipStream_t* xg_stream_t;
+
+#define XG_MAX_RANK 8
+#define XG_MAX_NODES 96
+#define XG_MAX_LEAVES 16
+#define XG_MAX_CONSTS 32
+
+/* ---- status codes ------------------------------------------------------ */
+typedef enum xg_status {
+    XG_OK = 0,
+    XG_BROADCAST_ERROR = 1,  /* xt::broadcast_error: incompatible shapes             */
+    XG_BAD_AXES = 2,         /* runtime_error: axes unsorted / duplicated / too big */
+    XG_UNSUPPORTED = 3,      /* valid xtensor expression this library cannot run;
+                                the C++ layer falls back to the host path         */
+    XG_INVALID_ARGUMENT = 4, /* malformed descriptor                                */
+    XG_TYPE_ERROR = 5,       /* operator not defined for the operand types
+                                (the xtensor expression would not compile)          */
+    XG_DEVICE_ERROR = 6,     /* a HIP call failed                                   */
+    XG_OUT_OF_MEMORY = 7,    /* workspace or admission budget exhausted             */
+    XG_BUSY = 8              /* server queue full                                   */
+} xg_status;
+
+/* ---- element types ------------------------------------------------------ */
+/* C++ types used by xtensor containers; BFLOAT16 is a storage-only type that
+ * loads as float (xtensor itself has no bf16). */
+typedef enum xg_dtype {
+    XG_BOOL = 0,
+    XG_INT8 = 1,
+    XG_UINT8 = 2,
+    XG_INT16 = 3,
+    XG_UINT16 = 4,
+    XG_INT32 = 5,
+    XG_UINT32 = 6,
+    XG_INT64 = 7,
+    XG_UINT64 = 8,
+    XG_FLOAT32 = 9,
+    XG_FLOAT64 = 10,
+    XG_BFLOAT16 = 11,
+    XG_NUM_DTYPES = 12
+} xg_dtype;
+
+/* ---- a strided array view ---------------------------------------------- */
+/* Shape and strides are in elements, row-major or any other order.  As in
+ * xtensor containers, the stride of a size-1 dimension is ignored (treated as
+ * 0).  Element (i0..in-1) lives at data + (offset + sum_k i_k*strides[k]) *
+ * sizeof(dtype).  Strides may be negative (reversed slices). */
+typedef struct xg_tensor {
+    void* data;
+    int32_t dtype;
+    int32_t ndim;
+    int64_t offset;
+    int64_t shape[XG_MAX_RANK];
+    int64_t strides[XG_MAX_RANK];
+} xg_tensor;
+
+/* ---- a typed scalar (xt::xscalar leaf, or a reducer's `initial`) -------- */
+typedef struct xg_scalar {
+    int32_t dtype;
+    int32_t pad_;
+    union {
+        int64_t i;   /* all signed integer types and bool */
+        uint64_t u;  /* all unsigned integer types        */
+        double f;    /* float32 (value exactly representable) and float64 */
+    } v;
+} xg_scalar;
+
+/* ---- expression opcodes (xtensor functors) ------------------------------ */
+typedef enum xg_op {
+    XG_OP_LEAF = 0,   /* index = leaf number                                  */
+    XG_OP_CONST = 1,  /* index = constant number                              */
+    /* unary operators, core/xoperation.hpp:86-103 */
+    XG_OP_NEGATE = 2,
+    XG_OP_IDENTITY = 3,
+    XG_OP_LOGICAL_NOT = 4,
+    XG_OP_BITWISE_NOT = 5,
+    XG_OP_CAST = 6,   /* xt::cast<T>, index = target dtype, core/xoperation.hpp:145-165 */
+    /* binary operators, core/xoperation.hpp:86-109 */
+    XG_OP_PLUS = 10,
+    XG_OP_MINUS = 11,
+    XG_OP_MULTIPLIES = 12,
+    XG_OP_DIVIDES = 13,
+    XG_OP_MODULUS = 14,
+    XG_OP_LOGICAL_AND = 15,
+    XG_OP_LOGICAL_OR = 16,
+    XG_OP_BITWISE_AND = 17,
+    XG_OP_BITWISE_OR = 18,
+    XG_OP_BITWISE_XOR = 19,
+    XG_OP_LEFT_SHIFT = 20,
+    XG_OP_RIGHT_SHIFT = 21,
+    XG_OP_LESS = 22,
+    XG_OP_LESS_EQUAL = 23,
+    XG_OP_GREATER = 24,
+    XG_OP_GREATER_EQUAL = 25,
+    XG_OP_EQUAL_TO = 26,
+    XG_OP_NOT_EQUAL_TO = 27,
+    XG_OP_MAXIMUM = 28, /* xt::maximum: select(a > b, a, b), core/xmath.hpp:589-602 */
+    XG_OP_MINIMUM = 29, /* xt::minimum: select(a < b, a, b), core/xmath.hpp:573-587 */
+    XG_OP_POW = 30,     /* std::pow, core/xmath.hpp:217 */
+    XG_OP_FMOD = 31,    /* std::fmod */
+    XG_OP_ATAN2 = 32,   /* std::atan2 */
+    XG_OP_HYPOT = 33,   /* std::hypot */
+    /* ternary */
+    XG_OP_WHERE = 40,   /* xt::where(c, a, b) = select(c, a, b), core/xoperation.hpp:775-784 */
+    XG_OP_CLIP = 41,    /* xt::clip(v, lo, hi), core/xmath.hpp:604-617, 814-823 */
+    /* unary math functions (std:: overloads), core/xmath.hpp:152-217 */
+    XG_OP_ABS = 50,
+    XG_OP_FABS = 51,
+    XG_OP_EXP = 52,
+    XG_OP_EXP2 = 53,
+    XG_OP_EXPM1 = 54,
+    XG_OP_LOG = 55,
+    XG_OP_LOG2 = 56,
+    XG_OP_LOG10 = 57,
+    XG_OP_LOG1P = 58,
+    XG_OP_SQRT = 59,
+    XG_OP_CBRT = 60,
+    XG_OP_SIN = 61,
+    XG_OP_COS = 62,
+    XG_OP_TAN = 63,
+    XG_OP_TANH = 64,
+    XG_OP_SINH = 65,
+    XG_OP_COSH = 66,
+    XG_OP_ASIN = 67,
+    XG_OP_ACOS = 68,
+    XG_OP_ATAN = 69,
+    XG_OP_ERF = 70,
+    XG_OP_FLOOR = 71,
+    XG_OP_CEIL = 72,
+    XG_OP_TRUNC = 73,
+    XG_OP_ROUND = 74
+} xg_op;
+
+/* One node of an expression tree.  Nodes are listed so that every argument
+ * precedes its user; the last node is the root.  A node may be used by
+ * several parents (a DAG): that evaluates to the same values as repeating the
+ * subtree, which is what xtensor would compute. */
+typedef struct xg_node {
+    int32_t op;       /* xg_op */
+    int32_t nargs;    /* 0..3 */
+    int32_t args[3];  /* node numbers of the arguments, in operator order */
+    int32_t index;    /* leaf / constant number, or target dtype for CAST */
+} xg_node;
+
+typedef struct xg_expr {
+    int32_t n_nodes;
+    int32_t n_leaves;
+    int32_t n_consts;
+    int32_t flags;           /* reserved, 0 */
+    const xg_node* nodes;
+    const xg_tensor* leaves; /* device views, in their own shapes          */
+    const xg_scalar* consts;
+} xg_expr;
+
+/* ---- reductions ---------------------------------------------------------- */
+typedef enum xg_reduce_op {
+    XG_RED_SUM = 0,       /* xt::sum<T>   (detail::plus, init 0)            */
+    XG_RED_PROD = 1,      /* xt::prod<T>  (detail::multiplies, init 1)      */
+    XG_RED_AMAX = 2,      /* xt::amax<T>  (maximum, init lowest())          */
+    XG_RED_AMIN = 3,      /* xt::amin<T>  (minimum, init max())             */
+    XG_RED_MEAN = 4,      /* xt::mean<T>: sum<T> / double(n)                */
+    XG_RED_NORM_SQ = 5,   /* xt::norm_sq: sum of v*v in big_promote type    */
+    XG_RED_NORM_L2 = 6,   /* xt::norm_l2 = sqrt(norm_sq)                    */
+    XG_RED_NORM_L1 = 7,   /* xt::norm_l1: sum of abs(v)                     */
+    XG_RED_NORM_LINF = 8, /* xt::norm_linf: max of abs(v)                   */
+    XG_RED_NORM_L0 = 9    /* xt::norm_l0: count of v != 0                   */
+} xg_reduce_op;
+
+typedef struct xg_reduce_desc {
+    int32_t op;           /* xg_reduce_op                                    */
+    int32_t acc_dtype;    /* the template argument T, or -1 for the default  */
+    int32_t n_axes;       /* -1 = all axes (the no-axes overloads)           */
+    int32_t axes[XG_MAX_RANK]; /* may be negative (normalised like xtensor) */
+    int32_t keep_dims;    /* xt::keep_dims option                            */
+    int32_t has_initial;  /* xt::initial(value) option                       */
+    xg_scalar initial;
+    int32_t flags;        /* reserved, 0                                      */
+    int32_t pad_;
+} xg_reduce_desc;
+
+/* ---- scans (accumulators) ------------------------------------------------ */
+typedef enum xg_scan_op {
+    XG_SCAN_CUMSUM = 0,
+    XG_SCAN_CUMPROD = 1
+} xg_scan_op;
+
+#define XG_AXIS_NONE (-1000) /* flattened scan (cumsum(e) without axis)     */
+
+typedef struct xg_scan_desc {
+    int32_t op;        /* xg_scan_op                                         */
+    int32_t acc_dtype; /* template argument T, or -1                          */
+    int32_t axis;      /* may be negative; XG_AXIS_NONE = flattened           */
+    int32_t flags;
+} xg_scan_desc;
+
+/* ---- library ------------------------------------------------------------- */
+
+/* Version string of this library. */
+const char* xg_version(void);
+
+/* Message of the last error raised on the calling thread ("" if none). */
+const char* xg_last_error_message(void);
+
+/* Initialise the library on the current HIP device (queries the device,
+ * allocates the per-device workspace pool).  Optional: every entry point
+ * initialises lazily. */
+xg_status xg_init(void);
+
+/* Shape and value type of an expression: broadcast shape of all leaves
+ * (xfunction::broadcast_shape), and the C++ value type of the root node
+ * (decltype of the functor applied to the children's value types).  Pure
+ * host computation, no device access. */
+xg_status xg_expr_info(const xg_expr* e, int32_t* dtype, int32_t* ndim, int64_t* shape);
+
+/* out = e   (xt::noalias(out) = e, or assignment into a strided view).
+ * The broadcast shape of e must be broadcastable to out's shape
+ * (xexpression_assigner::assert_compatible_shape).  Values are converted to
+ * out's dtype with static_cast (conditional_cast).  Outputs may alias inputs
+ * only element-for-element (out[i] read and written by the same element). */
+xg_status xg_assign(const xg_expr* e, const xg_tensor* out, xg_stream_t stream);
+
+/* Result dtype and shape of a reduction of e. */
+xg_status xg_reduce_info(const xg_expr* e, const xg_reduce_desc* r, int32_t* dtype, int32_t* ndim,
+                         int64_t* shape);
+
+/* out = reduce(e, axes)   (xt::sum/prod/amax/amin/mean/norm_* assigned into out).
+ * out must have exactly the reduction's shape; its dtype may differ from the
+ * result dtype (then values are static_cast on store). */
+xg_status xg_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, xg_stream_t stream);
+
+/* Result dtype and shape of a scan of e. */
+xg_status xg_scan_info(const xg_expr* e, const xg_scan_desc* s, int32_t* dtype, int32_t* ndim, int64_t* shape);
+
+/* out = cumsum/cumprod(e, axis). */
+xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, xg_stream_t stream);
+
+/* ---- prepared plans --------------------------------------------------------
+ * Plan once on the host, launch many times: a plan captures the normalised
+ * iteration space, the lowered program and the kernel choice for the given
+ * descriptors (including their device pointers).  Executing a plan only
+ * enqueues its kernels, so it may be called inside HIP graph capture.  No
+ * counterpart in xtensor (whose assignment is synchronous and re-derived on
+ * every call); it serves the batching and serving layers. */
+typedef struct xg_plan_s* xg_plan_t;
+
+xg_status xg_plan_assign(const xg_expr* e, const xg_tensor* out, xg_plan_t* plan);
+xg_status xg_plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, xg_plan_t* plan);
+xg_status xg_plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, xg_plan_t* plan);
+/* Bytes of device workspace the plan needs (0 for most elementwise plans). */
+size_t xg_plan_workspace_size(xg_plan_t plan);
+/* Enqueue the plan on `stream`; `workspace` must hold xg_plan_workspace_size
+ * bytes (NULL: the library's per-stream workspace is used). */
+xg_status xg_plan_execute(xg_plan_t plan, void* workspace, xg_stream_t stream);
+void xg_plan_destroy(xg_plan_t plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XG_H */

@@ -1,0 +1,1044 @@
+// oracle.cpp — CPU restatement of xtensor's lazy-expression evaluation.
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in the product (xtensor_amd/) links,
+// loads or calls this code; it is the checker that tests/, the smoke test
+// and the CPU-baseline leg of bench.py use.  It reads the same `xg_expr`
+// boundary format as the device library (include/xg.h), but derives every
+// semantic rule on its own, from xtensor 0.27.1:
+//
+//   * value types: each node's type is decltype(functor(args...)) computed by
+//     the C++ compiler itself, exactly as xfunction does
+//     (core/xfunction.hpp:133-142).  Functors restate core/xoperation.hpp:
+//     86-109 (operators), 775-784 (where), 145-165 (cast) and
+//     core/xmath.hpp:152-217 (std:: math), 573-617 (minimum/maximum/clamp).
+//     xtl::select is restated as `cond ? a : b` returning
+//     std::common_type_t<A,B> (xtl 0.8.0 xfunctional.hpp; xtl is not vendored,
+//     so that one rule is restated from xtl's published source).
+//   * broadcasting: right-aligned fold of every leaf shape in tree order into
+//     a shape of rank = max leaf rank (core/xstrides.hpp:737-780,
+//     core/xfunction.hpp:537-544, 709-727); size-1 dims read with stride 0
+//     (core/xstrides.hpp:503-530); assignment checks broadcastable()
+//     (core/xassign.hpp:539-550, core/xstrides.hpp:783-794).
+//   * store: static_cast to the destination type (utils/xutils.hpp:617-646).
+//   * reductions: xreducer_stepper::aggregate / aggregate_impl order —
+//     a sequential left fold along the innermost reduced axis, merged
+//     sequentially across outer reduced axes, `initial` merged last
+//     (reducers/xreducer.hpp:1778-1828); value type
+//     decltype(reduce(init_type, value_type)); sum/prod/amax/amin inits from
+//     core/xmath.hpp:367-423, 777-800, 1803-1823; mean = sum<T> / double(n)
+//     with n = e.size()/s.size() (core/xmath.hpp:1825-1874); norms from
+//     reducers/xnorm.hpp:225-290, 397-426, 493-526.
+//   * scans: copy then the in-place sequential recurrence
+//     res[k] = f(res[k-1], res[k]) along the axis
+//     (reducers/xaccumulator.hpp:215-297, 343-367), result type
+//     decltype(f(init_type, value_type)).
+//
+// Built with -ffp-contract=off -fwrapv so that every operator rounds (and
+// wraps) on its own, as the GPU kernels are built.
+//
+// Parity status of each part is pinned in tests/test_oracle_golden.py against
+// the reference's own known-answer tests.  Unpinned (no reference test fixes
+// them and the reference cannot be compiled here): xtl::big_promote_type_t
+// (accumulator type of norm_sq / norm_l1), restated as double for floating
+// types, int64 for signed and uint64 for unsigned integers.
+
+#include "../include/xg.h"
+
+#include <algorithm>
+#include <functional>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& m) { throw Fail{code, m}; }
+
+// ---------------------------------------------------------------- types
+template <class T> struct tag { using type = T; };
+
+template <class T> constexpr int dt_of() {
+    if constexpr (std::is_same_v<T, bool>) return XG_BOOL;
+    else if constexpr (std::is_same_v<T, int8_t>) return XG_INT8;
+    else if constexpr (std::is_same_v<T, uint8_t>) return XG_UINT8;
+    else if constexpr (std::is_same_v<T, int16_t>) return XG_INT16;
+    else if constexpr (std::is_same_v<T, uint16_t>) return XG_UINT16;
+    else if constexpr (std::is_same_v<T, int32_t>) return XG_INT32;
+    else if constexpr (std::is_same_v<T, uint32_t>) return XG_UINT32;
+    else if constexpr (std::is_same_v<T, int64_t> || std::is_same_v<T, long long>) return XG_INT64;
+    else if constexpr (std::is_same_v<T, uint64_t> || std::is_same_v<T, unsigned long long>) return XG_UINT64;
+    else if constexpr (std::is_same_v<T, float>) return XG_FLOAT32;
+    else if constexpr (std::is_same_v<T, double>) return XG_FLOAT64;
+    else return -1;
+}
+
+struct Val {
+    int dt = XG_BOOL;
+    union {
+        bool b;
+        int8_t i8;
+        uint8_t u8;
+        int16_t i16;
+        uint16_t u16;
+        int32_t i32;
+        uint32_t u32;
+        int64_t i64;
+        uint64_t u64;
+        float f32;
+        double f64;
+    };
+    Val() : i64(0) {}
+};
+
+template <class T> T get(const Val& v) {
+    if constexpr (std::is_same_v<T, bool>) return v.b;
+    else if constexpr (std::is_same_v<T, int8_t>) return v.i8;
+    else if constexpr (std::is_same_v<T, uint8_t>) return v.u8;
+    else if constexpr (std::is_same_v<T, int16_t>) return v.i16;
+    else if constexpr (std::is_same_v<T, uint16_t>) return v.u16;
+    else if constexpr (std::is_same_v<T, int32_t>) return v.i32;
+    else if constexpr (std::is_same_v<T, uint32_t>) return v.u32;
+    else if constexpr (dt_of<T>() == XG_INT64) return static_cast<T>(v.i64);
+    else if constexpr (dt_of<T>() == XG_UINT64) return static_cast<T>(v.u64);
+    else if constexpr (std::is_same_v<T, float>) return v.f32;
+    else return v.f64;
+}
+
+template <class T> Val make(T x) {
+    Val v;
+    v.dt = dt_of<T>();
+    if constexpr (std::is_same_v<T, bool>) v.b = x;
+    else if constexpr (std::is_same_v<T, int8_t>) v.i8 = x;
+    else if constexpr (std::is_same_v<T, uint8_t>) v.u8 = x;
+    else if constexpr (std::is_same_v<T, int16_t>) v.i16 = x;
+    else if constexpr (std::is_same_v<T, uint16_t>) v.u16 = x;
+    else if constexpr (std::is_same_v<T, int32_t>) v.i32 = x;
+    else if constexpr (std::is_same_v<T, uint32_t>) v.u32 = x;
+    else if constexpr (dt_of<T>() == XG_INT64) v.i64 = static_cast<int64_t>(x);
+    else if constexpr (dt_of<T>() == XG_UINT64) v.u64 = static_cast<uint64_t>(x);
+    else if constexpr (std::is_same_v<T, float>) v.f32 = x;
+    else v.f64 = x;
+    return v;
+}
+
+template <class F> decltype(auto) visit(int dt, F&& f) {
+    switch (dt) {
+    case XG_BOOL: return f(tag<bool>{});
+    case XG_INT8: return f(tag<int8_t>{});
+    case XG_UINT8: return f(tag<uint8_t>{});
+    case XG_INT16: return f(tag<int16_t>{});
+    case XG_UINT16: return f(tag<uint16_t>{});
+    case XG_INT32: return f(tag<int32_t>{});
+    case XG_UINT32: return f(tag<uint32_t>{});
+    case XG_INT64: return f(tag<int64_t>{});
+    case XG_UINT64: return f(tag<uint64_t>{});
+    case XG_FLOAT32: return f(tag<float>{});
+    case XG_FLOAT64: return f(tag<double>{});
+    default: fail(XG_INVALID_ARGUMENT, "bad dtype " + std::to_string(dt));
+    }
+}
+
+// storage: bf16 leaves load as float; other dtypes load as themselves
+int value_dt_of_storage(int dt) { return dt == XG_BFLOAT16 ? XG_FLOAT32 : dt; }
+
+size_t dt_size(int dt) {
+    switch (dt) {
+    case XG_BOOL: case XG_INT8: case XG_UINT8: return 1;
+    case XG_INT16: case XG_UINT16: case XG_BFLOAT16: return 2;
+    case XG_INT32: case XG_UINT32: case XG_FLOAT32: return 4;
+    default: return 8;
+    }
+}
+
+float bf16_to_f32(uint16_t h) {
+    uint32_t u = uint32_t(h) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+uint16_t f32_to_bf16(float f) {  // round to nearest even, NaN kept quiet
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40);
+    uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(r >> 16);
+}
+
+Val load(const xg_tensor& t, int64_t elem) {
+    const char* p = static_cast<const char*>(t.data) + elem * int64_t(dt_size(t.dtype));
+    if (t.dtype == XG_BFLOAT16) {
+        uint16_t h;
+        std::memcpy(&h, p, 2);
+        return make<float>(bf16_to_f32(h));
+    }
+    return visit(t.dtype, [&](auto tg) {
+        using T = typename decltype(tg)::type;
+        T x;
+        std::memcpy(&x, p, sizeof(T));
+        return make<T>(x);
+    });
+}
+
+template <class T> T cast_to(const Val& v) {
+    return visit(v.dt, [&](auto tg) {
+        using S = typename decltype(tg)::type;
+        return static_cast<T>(get<S>(v));
+    });
+}
+
+Val convert(const Val& v, int dt) {
+    return visit(dt, [&](auto tg) {
+        using T = typename decltype(tg)::type;
+        return make<T>(cast_to<T>(v));
+    });
+}
+
+void store(const xg_tensor& t, int64_t elem, const Val& v) {
+    char* p = static_cast<char*>(t.data) + elem * int64_t(dt_size(t.dtype));
+    if (t.dtype == XG_BFLOAT16) {
+        uint16_t h = f32_to_bf16(cast_to<float>(v));
+        std::memcpy(p, &h, 2);
+        return;
+    }
+    visit(t.dtype, [&](auto tg) {
+        using T = typename decltype(tg)::type;
+        T x = cast_to<T>(v);
+        std::memcpy(p, &x, sizeof(T));
+        return 0;
+    });
+}
+
+// ---------------------------------------------------------------- functors
+// xtl::select: cond ? v1 : v2 with result std::common_type_t<T1, T2>
+template <class B, class A1, class A2> auto xselect(const B& c, const A1& a, const A2& b) {
+    using R = std::common_type_t<A1, A2>;
+    return static_cast<R>(c ? a : b);
+}
+
+#define UN(NAME, EXPR)                                                           \
+    struct NAME {                                                                \
+        template <class A> static auto f(const A& a) -> decltype(EXPR) { return EXPR; } \
+    };
+#define BIN(NAME, EXPR)                                                                            \
+    struct NAME {                                                                                  \
+        template <class A, class B> static auto f(const A& a, const B& b) -> decltype(EXPR) { return EXPR; } \
+    };
+
+UN(f_negate, -a)
+UN(f_identity, +a)
+UN(f_lnot, !a)
+UN(f_bnot, ~a)
+BIN(f_plus, a + b)
+BIN(f_minus, a - b)
+BIN(f_mul, a * b)
+BIN(f_div, a / b)
+BIN(f_mod, a % b)
+BIN(f_land, a && b)
+BIN(f_lor, a || b)
+BIN(f_band, a & b)
+BIN(f_bor, a | b)
+BIN(f_bxor, a ^ b)
+BIN(f_shl, a << b)
+BIN(f_shr, a >> b)
+BIN(f_lt, a < b)
+BIN(f_le, a <= b)
+BIN(f_gt, a > b)
+BIN(f_ge, a >= b)
+BIN(f_eq, a == b)
+BIN(f_ne, a != b)
+BIN(f_maximum, xselect(a > b, a, b))
+BIN(f_minimum, xselect(a < b, a, b))
+BIN(f_pow, std::pow(a, b))
+BIN(f_fmod, std::fmod(a, b))
+BIN(f_atan2, std::atan2(a, b))
+BIN(f_hypot, std::hypot(a, b))
+// math::abs: identity on unsigned integer types (core/xmath.hpp:59-63, 283-287), std::abs otherwise
+template <class A> auto xabs(const A& a) {
+    if constexpr (std::is_unsigned_v<A> && !std::is_same_v<A, bool>) return a;
+    else return std::abs(a);
+}
+UN(f_abs, xabs(a))
+UN(f_fabs, std::fabs(a))
+UN(f_exp, std::exp(a))
+UN(f_exp2, std::exp2(a))
+UN(f_expm1, std::expm1(a))
+UN(f_log, std::log(a))
+UN(f_log2, std::log2(a))
+UN(f_log10, std::log10(a))
+UN(f_log1p, std::log1p(a))
+UN(f_sqrt, std::sqrt(a))
+UN(f_cbrt, std::cbrt(a))
+UN(f_sin, std::sin(a))
+UN(f_cos, std::cos(a))
+UN(f_tan, std::tan(a))
+UN(f_tanh, std::tanh(a))
+UN(f_sinh, std::sinh(a))
+UN(f_cosh, std::cosh(a))
+UN(f_asin, std::asin(a))
+UN(f_acos, std::acos(a))
+UN(f_atan, std::atan(a))
+UN(f_erf, std::erf(a))
+UN(f_floor, std::floor(a))
+UN(f_ceil, std::ceil(a))
+UN(f_trunc, std::trunc(a))
+UN(f_round, std::round(a))
+
+// bitwise operators on bool operands promote to int in C++; std math on bool
+// is not an xtensor use and is rejected like a compile error would.
+template <class F, class... A> constexpr bool callable = requires(A... a) { F::f(a...); };
+
+template <class F> Val apply1(const Val& a) {
+    return visit(a.dt, [&](auto ta) -> Val {
+        using A = typename decltype(ta)::type;
+        if constexpr (callable<F, A>) {
+            auto r = F::f(get<A>(a));
+            using R = decltype(r);
+            if constexpr (dt_of<R>() < 0) fail(XG_TYPE_ERROR, "unsupported result type");
+            else return make<R>(r);
+        } else {
+            fail(XG_TYPE_ERROR, "operator not defined for operand type");
+        }
+    });
+}
+
+template <class F> Val apply2(const Val& a, const Val& b) {
+    return visit(a.dt, [&](auto ta) -> Val {
+        using A = typename decltype(ta)::type;
+        return visit(b.dt, [&](auto tb) -> Val {
+            using B = typename decltype(tb)::type;
+            if constexpr (callable<F, A, B>) {
+                auto r = F::f(get<A>(a), get<B>(b));
+                using R = decltype(r);
+                if constexpr (dt_of<R>() < 0) fail(XG_TYPE_ERROR, "unsupported result type");
+                else return make<R>(r);
+            } else {
+                fail(XG_TYPE_ERROR, "operator not defined for operand types");
+            }
+        });
+    });
+}
+
+// std::common_type_t of two value types
+int common_dt(int a, int b) {
+    return visit(a, [&](auto ta) {
+        using A = typename decltype(ta)::type;
+        return visit(b, [&](auto tb) {
+            using B = typename decltype(tb)::type;
+            return dt_of<std::common_type_t<A, B>>();
+        });
+    });
+}
+
+bool truth(const Val& v) { return cast_to<bool>(v); }
+
+// xtl::select(cond, a, b): the chosen operand converted to common_type<A, B>
+Val select_dyn(bool c, const Val& a, const Val& b) { return convert(c ? a : b, common_dt(a.dt, b.dt)); }
+
+// clamp_fun, core/xmath.hpp:604-617:
+// select(lo < hi, select(v < lo, lo, select(hi < v, hi, v)), hi)
+Val clip_dyn(const Val& v, const Val& lo, const Val& hi) {
+    Val inner = select_dyn(truth(apply2<f_lt>(hi, v)), hi, v);
+    Val mid = select_dyn(truth(apply2<f_lt>(v, lo)), lo, inner);
+    return select_dyn(truth(apply2<f_lt>(lo, hi)), mid, hi);
+}
+
+// integer division / modulus by zero is undefined in C++; the oracle reports
+// it instead of trapping, so tests never feed it.
+void check_div(int op, const Val& b) {
+    if ((op == XG_OP_DIVIDES || op == XG_OP_MODULUS) && b.dt != XG_FLOAT32 && b.dt != XG_FLOAT64) {
+        if (cast_to<int64_t>(b) == 0 && cast_to<uint64_t>(b) == 0)
+            fail(XG_INVALID_ARGUMENT, "integer division by zero");
+    }
+}
+
+Val eval_op(const xg_node& n, const Val* a) {
+    switch (n.op) {
+    case XG_OP_NEGATE: return apply1<f_negate>(a[0]);
+    case XG_OP_IDENTITY: return apply1<f_identity>(a[0]);
+    case XG_OP_LOGICAL_NOT: return apply1<f_lnot>(a[0]);
+    case XG_OP_BITWISE_NOT: return apply1<f_bnot>(a[0]);
+    case XG_OP_CAST: return convert(a[0], n.index);
+    case XG_OP_PLUS: return apply2<f_plus>(a[0], a[1]);
+    case XG_OP_MINUS: return apply2<f_minus>(a[0], a[1]);
+    case XG_OP_MULTIPLIES: return apply2<f_mul>(a[0], a[1]);
+    case XG_OP_DIVIDES: check_div(n.op, a[1]); return apply2<f_div>(a[0], a[1]);
+    case XG_OP_MODULUS: check_div(n.op, a[1]); return apply2<f_mod>(a[0], a[1]);
+    case XG_OP_LOGICAL_AND: return apply2<f_land>(a[0], a[1]);
+    case XG_OP_LOGICAL_OR: return apply2<f_lor>(a[0], a[1]);
+    case XG_OP_BITWISE_AND: return apply2<f_band>(a[0], a[1]);
+    case XG_OP_BITWISE_OR: return apply2<f_bor>(a[0], a[1]);
+    case XG_OP_BITWISE_XOR: return apply2<f_bxor>(a[0], a[1]);
+    case XG_OP_LEFT_SHIFT: return apply2<f_shl>(a[0], a[1]);
+    case XG_OP_RIGHT_SHIFT: return apply2<f_shr>(a[0], a[1]);
+    case XG_OP_LESS: return apply2<f_lt>(a[0], a[1]);
+    case XG_OP_LESS_EQUAL: return apply2<f_le>(a[0], a[1]);
+    case XG_OP_GREATER: return apply2<f_gt>(a[0], a[1]);
+    case XG_OP_GREATER_EQUAL: return apply2<f_ge>(a[0], a[1]);
+    case XG_OP_EQUAL_TO: return apply2<f_eq>(a[0], a[1]);
+    case XG_OP_NOT_EQUAL_TO: return apply2<f_ne>(a[0], a[1]);
+    case XG_OP_MAXIMUM: return apply2<f_maximum>(a[0], a[1]);
+    case XG_OP_MINIMUM: return apply2<f_minimum>(a[0], a[1]);
+    case XG_OP_POW: return apply2<f_pow>(a[0], a[1]);
+    case XG_OP_FMOD: return apply2<f_fmod>(a[0], a[1]);
+    case XG_OP_ATAN2: return apply2<f_atan2>(a[0], a[1]);
+    case XG_OP_HYPOT: return apply2<f_hypot>(a[0], a[1]);
+    case XG_OP_WHERE: return select_dyn(truth(a[0]), a[1], a[2]);
+    case XG_OP_CLIP: return clip_dyn(a[0], a[1], a[2]);
+    case XG_OP_ABS: return apply1<f_abs>(a[0]);
+    case XG_OP_FABS: return apply1<f_fabs>(a[0]);
+    case XG_OP_EXP: return apply1<f_exp>(a[0]);
+    case XG_OP_EXP2: return apply1<f_exp2>(a[0]);
+    case XG_OP_EXPM1: return apply1<f_expm1>(a[0]);
+    case XG_OP_LOG: return apply1<f_log>(a[0]);
+    case XG_OP_LOG2: return apply1<f_log2>(a[0]);
+    case XG_OP_LOG10: return apply1<f_log10>(a[0]);
+    case XG_OP_LOG1P: return apply1<f_log1p>(a[0]);
+    case XG_OP_SQRT: return apply1<f_sqrt>(a[0]);
+    case XG_OP_CBRT: return apply1<f_cbrt>(a[0]);
+    case XG_OP_SIN: return apply1<f_sin>(a[0]);
+    case XG_OP_COS: return apply1<f_cos>(a[0]);
+    case XG_OP_TAN: return apply1<f_tan>(a[0]);
+    case XG_OP_TANH: return apply1<f_tanh>(a[0]);
+    case XG_OP_SINH: return apply1<f_sinh>(a[0]);
+    case XG_OP_COSH: return apply1<f_cosh>(a[0]);
+    case XG_OP_ASIN: return apply1<f_asin>(a[0]);
+    case XG_OP_ACOS: return apply1<f_acos>(a[0]);
+    case XG_OP_ATAN: return apply1<f_atan>(a[0]);
+    case XG_OP_ERF: return apply1<f_erf>(a[0]);
+    case XG_OP_FLOOR: return apply1<f_floor>(a[0]);
+    case XG_OP_CEIL: return apply1<f_ceil>(a[0]);
+    case XG_OP_TRUNC: return apply1<f_trunc>(a[0]);
+    case XG_OP_ROUND: return apply1<f_round>(a[0]);
+    default: fail(XG_INVALID_ARGUMENT, "unknown op " + std::to_string(n.op));
+    }
+}
+
+// ---------------------------------------------------------------- shapes
+using Shape = std::vector<int64_t>;
+constexpr int64_t UNSET = std::numeric_limits<int64_t>::max();
+
+// xt::broadcast_shape(input, output), core/xstrides.hpp:737-780
+void broadcast_into(const Shape& in, Shape& out) {
+    if (out.size() < in.size()) fail(XG_BROADCAST_ERROR, "broadcast: input rank exceeds output rank");
+    size_t oi = out.size(), ii = in.size();
+    for (; ii != 0; --ii, --oi) {
+        int64_t& o = out[oi - 1];
+        int64_t i = in[ii - 1];
+        if (o == UNSET) o = i;
+        else if (o == 1) o = i;
+        else if (i == 1) { /* keep */ }
+        else if (i != o) fail(XG_BROADCAST_ERROR, "incompatible dimension of inputs");
+    }
+}
+
+// xt::broadcastable(src, dst), core/xstrides.hpp:783-794
+bool broadcastable(const Shape& src, const Shape& dst) {
+    if (dst.size() < src.size()) return false;
+    for (size_t k = 0; k < src.size(); ++k) {
+        int64_t s = src[src.size() - 1 - k], d = dst[dst.size() - 1 - k];
+        if (!(s == d || s == 1)) return false;
+    }
+    return true;
+}
+
+struct Tree {
+    const xg_expr* e;
+    std::vector<int> node_dt;   // value type of each node
+    std::vector<int> node_rank; // dimension() of each node
+    Shape shape;                // broadcast shape of the root
+};
+
+void check_expr(const xg_expr* e) {
+    if (!e || e->n_nodes <= 0 || e->n_nodes > XG_MAX_NODES || e->n_leaves < 0 || e->n_leaves > XG_MAX_LEAVES ||
+        e->n_consts < 0 || e->n_consts > XG_MAX_CONSTS)
+        fail(XG_INVALID_ARGUMENT, "bad expression header");
+    for (int i = 0; i < e->n_nodes; ++i) {
+        const xg_node& n = e->nodes[i];
+        for (int k = 0; k < n.nargs; ++k)
+            if (n.args[k] < 0 || n.args[k] >= i) fail(XG_INVALID_ARGUMENT, "node argument out of order");
+        if (n.op == XG_OP_LEAF && (n.index < 0 || n.index >= e->n_leaves)) fail(XG_INVALID_ARGUMENT, "bad leaf");
+        if (n.op == XG_OP_CONST && (n.index < 0 || n.index >= e->n_consts)) fail(XG_INVALID_ARGUMENT, "bad const");
+    }
+    for (int l = 0; l < e->n_leaves; ++l)
+        if (e->leaves[l].ndim < 0 || e->leaves[l].ndim > XG_MAX_RANK) fail(XG_INVALID_ARGUMENT, "bad leaf rank");
+}
+
+Val const_val(const xg_scalar& s) {
+    return visit(s.dtype, [&](auto tg) {
+        using T = typename decltype(tg)::type;
+        if constexpr (std::is_floating_point_v<T>) return make<T>(static_cast<T>(s.v.f));
+        else if constexpr (std::is_signed_v<T>) return make<T>(static_cast<T>(s.v.i));
+        else return make<T>(static_cast<T>(s.v.u));
+    });
+}
+
+// collect leaves in tree (DFS, argument) order, as xfunction::broadcast_shape visits them
+void dfs_leaves(const xg_expr* e, int node, std::vector<int>& order) {
+    const xg_node& n = e->nodes[node];
+    if (n.op == XG_OP_LEAF) { order.push_back(n.index); return; }
+    for (int k = 0; k < n.nargs; ++k) dfs_leaves(e, n.args[k], order);
+}
+
+Tree analyse(const xg_expr* e) {
+    check_expr(e);
+    Tree t;
+    t.e = e;
+    t.node_dt.resize(e->n_nodes);
+    t.node_rank.resize(e->n_nodes);
+    // value types: evaluate the functor on zero-valued samples of the argument types
+    for (int i = 0; i < e->n_nodes; ++i) {
+        const xg_node& n = e->nodes[i];
+        if (n.op == XG_OP_LEAF) {
+            t.node_dt[i] = value_dt_of_storage(e->leaves[n.index].dtype);
+            t.node_rank[i] = e->leaves[n.index].ndim;
+        } else if (n.op == XG_OP_CONST) {
+            t.node_dt[i] = e->consts[n.index].dtype;
+            t.node_rank[i] = 0;
+        } else {
+            Val a[3];
+            int r = 0;
+            for (int k = 0; k < n.nargs; ++k) {
+                a[k] = convert(make<int32_t>(1), t.node_dt[n.args[k]]);
+                r = std::max(r, t.node_rank[n.args[k]]);
+            }
+            // use 1 to avoid division by zero in the type probe
+            t.node_dt[i] = eval_op(n, a).dt;
+            t.node_rank[i] = r;
+        }
+    }
+    int root = e->n_nodes - 1;
+    t.shape.assign(size_t(t.node_rank[root]), UNSET);
+    std::vector<int> order;
+    dfs_leaves(e, root, order);
+    for (int l : order) {
+        Shape s(e->leaves[l].shape, e->leaves[l].shape + e->leaves[l].ndim);
+        broadcast_into(s, t.shape);
+    }
+    return t;
+}
+
+// per-leaf strides against an iteration shape: missing leading dims and
+// size-1 dims read with stride 0 (xstepper offset + adapt_strides)
+std::vector<std::vector<int64_t>> leaf_strides(const xg_expr* e, const Shape& iter) {
+    std::vector<std::vector<int64_t>> st(e->n_leaves, std::vector<int64_t>(iter.size(), 0));
+    for (int l = 0; l < e->n_leaves; ++l) {
+        const xg_tensor& t = e->leaves[l];
+        int off = int(iter.size()) - t.ndim;
+        for (int k = 0; k < t.ndim; ++k) {
+            if (off + k < 0) continue;
+            st[l][off + k] = (t.shape[k] == 1) ? 0 : t.strides[k];
+        }
+    }
+    return st;
+}
+
+int64_t numel(const Shape& s) {
+    int64_t n = 1;
+    for (auto x : s) n *= x;
+    return n;
+}
+
+// evaluate the tree at one multi-index; vals sized n_nodes
+struct Evaluator {
+    const xg_expr* e;
+    std::vector<std::vector<int64_t>> st;
+    std::vector<Val> consts;
+    std::vector<Val> vals;
+    Evaluator(const xg_expr* ex, const Shape& iter) : e(ex), st(leaf_strides(ex, iter)), vals(ex->n_nodes) {
+        for (int c = 0; c < e->n_consts; ++c) consts.push_back(const_val(e->consts[c]));
+    }
+    const Val& at(const int64_t* idx, int rank) {
+        for (int i = 0; i < e->n_nodes; ++i) {
+            const xg_node& n = e->nodes[i];
+            if (n.op == XG_OP_LEAF) {
+                const xg_tensor& t = e->leaves[n.index];
+                int64_t off = t.offset;
+                for (int k = 0; k < rank; ++k) off += idx[k] * st[n.index][k];
+                vals[i] = load(t, off);
+            } else if (n.op == XG_OP_CONST) {
+                vals[i] = consts[n.index];
+            } else {
+                Val a[3];
+                for (int k = 0; k < n.nargs; ++k) a[k] = vals[n.args[k]];
+                vals[i] = eval_op(n, a);
+            }
+        }
+        return vals[e->n_nodes - 1];
+    }
+};
+
+bool next_index(std::vector<int64_t>& idx, const Shape& shape) {
+    for (int k = int(shape.size()) - 1; k >= 0; --k) {
+        if (++idx[k] < shape[k]) return true;
+        idx[k] = 0;
+    }
+    return false;
+}
+
+int64_t out_offset(const xg_tensor& o, const int64_t* idx) {
+    int64_t off = o.offset;
+    for (int k = 0; k < o.ndim; ++k) off += idx[k] * (o.shape[k] == 1 ? 0 : o.strides[k]);
+    return off;
+}
+
+void do_assign(const xg_expr* e, const xg_tensor* out) {
+    Tree t = analyse(e);
+    Shape os(out->shape, out->shape + out->ndim);
+    if (!broadcastable(t.shape, os)) fail(XG_BROADCAST_ERROR, "cannot broadcast expression into output");
+    if (numel(os) == 0) return;
+    Evaluator ev(e, os);
+    std::vector<int64_t> idx(os.size(), 0);
+    do {
+        const Val& v = ev.at(idx.data(), int(os.size()));
+        store(*out, out_offset(*out, idx.data()), v);
+    } while (next_index(idx, os));
+}
+
+// ---------------------------------------------------------------- reductions
+template <class T> struct big_promote {  // unpinned: xtl::big_promote_type_t
+    using type = std::conditional_t<std::is_floating_point_v<T>, double,
+                                    std::conditional_t<std::is_signed_v<T>, int64_t, uint64_t>>;
+};
+
+// scalar norms of reducers/xnorm.hpp:200-290
+template <class T> auto nabs(T t) {
+    if constexpr (std::is_unsigned_v<T>) return t;
+    else return std::abs(t);
+}
+
+struct RedSpec {
+    int op;
+    int acc;  // T or -1
+};
+
+struct ReduceResult {
+    int dt;
+    Shape shape;
+};
+
+std::vector<int> normalise_axes(const xg_reduce_desc* r, int ndim) {
+    std::vector<int> ax;
+    if (r->n_axes < 0) {
+        for (int k = 0; k < ndim; ++k) ax.push_back(k);
+        return ax;
+    }
+    if (r->n_axes > XG_MAX_RANK) fail(XG_INVALID_ARGUMENT, "too many axes");
+    for (int k = 0; k < r->n_axes; ++k) {
+        int a = r->axes[k];
+        if (a < 0) a += ndim;  // xt::forward_normalize
+        if (a < 0) fail(XG_BAD_AXES, "Axis out of bounds for reduction.");
+        ax.push_back(a);
+    }
+    // reducers/xreducer.hpp:1426-1440
+    if (!std::is_sorted(ax.begin(), ax.end())) fail(XG_BAD_AXES, "Reducing axes should be sorted.");
+    if (std::adjacent_find(ax.begin(), ax.end()) != ax.end()) fail(XG_BAD_AXES, "Reducing axes should not contain duplicates.");
+    if (!ax.empty() && ax.back() > ndim - 1)
+        fail(XG_BAD_AXES, "Axis " + std::to_string(ax.back()) + " out of bounds for reduction.");
+    return ax;
+}
+
+Shape reduce_shape(const Shape& in, const std::vector<int>& ax, bool keep) {
+    Shape s;
+    for (int k = 0; k < int(in.size()); ++k) {
+        bool red = std::find(ax.begin(), ax.end(), k) != ax.end();
+        if (!red) s.push_back(in[k]);
+        else if (keep) s.push_back(1);
+    }
+    return s;
+}
+
+// Generic sequential reduction over `ax` for every output element.
+// VAL: accumulator value type; fn(acc, x) and mg(acc, acc) produce VAL.
+template <class VAL, class FN, class MG>
+void reduce_loop(Evaluator& ev, const Shape& in, const std::vector<int>& ax, bool keep, VAL init, bool has_initial,
+                 VAL initial, FN fn, MG mg, const std::function<void(int64_t, VAL)>& emit) {
+    int nd = int(in.size());
+    std::vector<int> kept;
+    for (int k = 0; k < nd; ++k)
+        if (std::find(ax.begin(), ax.end(), k) == ax.end()) kept.push_back(k);
+    Shape kshape;
+    for (int k : kept) kshape.push_back(in[k]);
+    int64_t nout = numel(kshape);
+    int64_t total = numel(in);
+    VAL start = has_initial ? initial : init;  // initial_value()
+    std::vector<int64_t> idx(size_t(nd), 0);
+    std::vector<int64_t> kidx(kept.size(), 0);
+    for (int64_t o = 0; o < nout; ++o) {
+        for (size_t j = 0; j < kept.size(); ++j) idx[size_t(kept[j])] = kidx[j];
+        VAL res;
+        if (total == 0) {
+            res = start;
+        } else if (nd == 0 || ax.empty()) {
+            res = fn(start, ev.at(idx.data(), nd));
+        } else {
+            // aggregate_impl(dim): recursion over the reduced axes in order
+            std::function<VAL(size_t)> agg = [&](size_t d) -> VAL {
+                int axis = ax[d];
+                int64_t n = in[size_t(axis)];
+                VAL r;
+                if (d + 1 != ax.size()) {
+                    idx[size_t(axis)] = 0;
+                    r = agg(d + 1);
+                    for (int64_t i = 1; i < n; ++i) {
+                        idx[size_t(axis)] = i;
+                        r = mg(r, agg(d + 1));
+                    }
+                } else {
+                    idx[size_t(axis)] = 0;
+                    r = fn(init, ev.at(idx.data(), nd));
+                    for (int64_t i = 1; i < n; ++i) {
+                        idx[size_t(axis)] = i;
+                        r = fn(r, ev.at(idx.data(), nd));
+                    }
+                }
+                idx[size_t(axis)] = 0;
+                return r;
+            };
+            res = agg(0);
+            if (has_initial) res = mg(initial, res);
+        }
+        emit(o, res);
+        // advance kept multi-index (row-major over the output)
+        for (int j = int(kept.size()) - 1; j >= 0; --j) {
+            if (++kidx[size_t(j)] < kshape[size_t(j)]) break;
+            kidx[size_t(j)] = 0;
+        }
+    }
+}
+
+// result dtype of a reduction with element value type `edt`
+int reduce_result_dt(int op, int acc, int edt) {
+    return visit(edt, [&](auto te) -> int {
+        using E = typename decltype(te)::type;
+        auto with_init = [&](auto ti) -> int {
+            using IT = typename decltype(ti)::type;
+            switch (op) {
+            case XG_RED_SUM: return dt_of<decltype(IT() + E())>();
+            case XG_RED_PROD: return dt_of<decltype(IT() * E())>();
+            case XG_RED_AMAX: return dt_of<decltype(xselect(IT() > E(), IT(), E()))>();
+            case XG_RED_AMIN: return dt_of<decltype(xselect(IT() < E(), IT(), E()))>();
+            case XG_RED_MEAN: {
+                using S = decltype(IT() + E());
+                return dt_of<decltype(S() / double())>();
+            }
+            default: break;
+            }
+            return -1;
+        };
+        if (op <= XG_RED_MEAN) {
+            if (acc >= 0) return visit(acc, with_init);
+            return with_init(te);
+        }
+        using BP = typename big_promote<E>::type;
+        switch (op) {
+        case XG_RED_NORM_SQ: return dt_of<decltype(BP() + E() * E())>();
+        case XG_RED_NORM_L2: return dt_of<decltype(std::sqrt(decltype(BP() + E() * E())()))>();
+        case XG_RED_NORM_L1: return dt_of<decltype(BP() + nabs(E()))>();
+        case XG_RED_NORM_LINF: return dt_of<decltype(nabs(E()))>();
+        case XG_RED_NORM_L0: return XG_UINT64;
+        default: fail(XG_INVALID_ARGUMENT, "bad reduce op");
+        }
+    });
+}
+
+void do_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, bool high_precision) {
+    Tree t = analyse(e);
+    const Shape& in = t.shape;
+    std::vector<int> ax = normalise_axes(r, int(in.size()));
+    bool keep = r->keep_dims != 0;
+    Shape rs = reduce_shape(in, ax, keep);
+    Shape os(out->shape, out->shape + out->ndim);
+    if (os != rs) fail(XG_BROADCAST_ERROR, "output shape does not match reduction shape");
+    int edt = t.node_dt[size_t(e->n_nodes - 1)];
+    Evaluator ev(e, in);
+    std::vector<int64_t> kept_strides;  // output addressing (row-major over the kept dims)
+    // output element o (row-major over the reduction shape) -> out offset
+    auto emit_offset = [&](int64_t o) {
+        int64_t off = out->offset;
+        for (int k = out->ndim - 1; k >= 0; --k) {
+            int64_t s = out->shape[k];
+            int64_t i = o % s;
+            o /= s;
+            off += i * (s == 1 ? 0 : out->strides[k]);
+        }
+        return off;
+    };
+    if (high_precision) {
+        // long-double accumulation of the element values for error statements
+        using LD = long double;
+        LD init = 0, ini = 0;
+        int op = r->op;
+        if (op == XG_RED_PROD) init = 1;
+        if (op == XG_RED_AMAX) init = -std::numeric_limits<LD>::infinity();
+        if (op == XG_RED_AMIN) init = std::numeric_limits<LD>::infinity();
+        if (r->has_initial) ini = cast_to<double>(const_val(r->initial));
+        auto told = [&](const Val& v) { return LD(cast_to<double>(v)); };
+        auto fn = [&](LD acc, const Val& v) -> LD {
+            LD x = told(v);
+            switch (op) {
+            case XG_RED_SUM: case XG_RED_MEAN: return acc + x;
+            case XG_RED_PROD: return acc * x;
+            case XG_RED_AMAX: return acc > x ? acc : x;
+            case XG_RED_AMIN: return acc < x ? acc : x;
+            case XG_RED_NORM_SQ: case XG_RED_NORM_L2: return acc + x * x;
+            case XG_RED_NORM_L1: return acc + (x < 0 ? -x : x);
+            case XG_RED_NORM_LINF: return std::max(acc, x < 0 ? -x : x);
+            case XG_RED_NORM_L0: return acc + (x != 0);
+            }
+            return acc;
+        };
+        auto mg = [&](LD a, LD b) -> LD {
+            switch (op) {
+            case XG_RED_PROD: return a * b;
+            case XG_RED_AMAX: return a > b ? a : b;
+            case XG_RED_AMIN: return a < b ? a : b;
+            case XG_RED_NORM_LINF: return std::max(a, b);
+            default: return a + b;
+            }
+        };
+        int64_t n_red = 1;
+        for (int a : ax) n_red *= in[size_t(a)];
+        reduce_loop<LD>(ev, in, ax, keep, init, r->has_initial != 0, ini, fn, mg, [&](int64_t o, LD v) {
+            if (op == XG_RED_MEAN) v = v / LD(n_red);
+            if (op == XG_RED_NORM_L2) v = std::sqrt(v);
+            store(*out, emit_offset(o), make<double>(double(v)));
+        });
+        return;
+    }
+    int rdt = reduce_result_dt(r->op, r->acc_dtype, edt);
+    if (rdt < 0) fail(XG_TYPE_ERROR, "reduction not defined for this type");
+    int op = r->op;
+    int64_t n_out = numel(rs);
+    // The reducer runs in its value type V: rdt for sum/prod/amax/amin and the
+    // norms, the sum's type for mean, the norm_sq type for norm_l2.  Each step
+    // applies the C++ functor to (V, element) and converts back to V, which
+    // is what assigning into xreducer_stepper's `reference res` does.
+    int vdt = rdt;
+    if (op == XG_RED_MEAN) vdt = reduce_result_dt(XG_RED_SUM, r->acc_dtype, edt);
+    if (op == XG_RED_NORM_L2) vdt = reduce_result_dt(XG_RED_NORM_SQ, -1, edt);
+    int idt = (op <= XG_RED_MEAN && r->acc_dtype >= 0) ? r->acc_dtype : edt;
+    Val init;
+    switch (op) {
+    case XG_RED_SUM: case XG_RED_MEAN: init = convert(convert(make<int32_t>(0), idt), vdt); break;
+    case XG_RED_PROD: init = convert(convert(make<int32_t>(1), idt), vdt); break;
+    case XG_RED_AMAX:
+        init = convert(visit(idt, [](auto t) { using T = typename decltype(t)::type; return make<T>(std::numeric_limits<T>::lowest()); }), vdt);
+        break;
+    case XG_RED_AMIN:
+        init = convert(visit(idt, [](auto t) { using T = typename decltype(t)::type; return make<T>(std::numeric_limits<T>::max()); }), vdt);
+        break;
+    default: init = convert(make<int32_t>(0), vdt); break;
+    }
+    Val ini = r->has_initial ? convert(const_val(r->initial), vdt) : init;
+    bool unsigned_e = (edt == XG_BOOL || edt == XG_UINT8 || edt == XG_UINT16 || edt == XG_UINT32 || edt == XG_UINT64);
+    auto nabs_dyn = [&](const Val& x) { return unsigned_e ? x : apply1<f_abs>(x); };
+    auto fn = [&](const Val& acc, const Val& x) -> Val {
+        switch (op) {
+        case XG_RED_SUM: case XG_RED_MEAN: return convert(apply2<f_plus>(acc, x), vdt);
+        case XG_RED_PROD: return convert(apply2<f_mul>(acc, x), vdt);
+        case XG_RED_AMAX: return convert(apply2<f_maximum>(acc, x), vdt);
+        case XG_RED_AMIN: return convert(apply2<f_minimum>(acc, x), vdt);
+        case XG_RED_NORM_SQ: case XG_RED_NORM_L2: return convert(apply2<f_plus>(acc, apply2<f_mul>(x, x)), vdt);
+        case XG_RED_NORM_L1: return convert(apply2<f_plus>(acc, nabs_dyn(x)), vdt);
+        case XG_RED_NORM_LINF: {  // std::max<result_type>(r, norm_linf(v))
+            Val a = convert(nabs_dyn(x), vdt);
+            return truth(apply2<f_lt>(acc, a)) ? a : acc;
+        }
+        case XG_RED_NORM_L0: return convert(apply2<f_plus>(acc, make<int32_t>(truth(x) ? 1 : 0)), vdt);
+        }
+        fail(XG_INVALID_ARGUMENT, "bad reduce op");
+    };
+    auto mg = [&](const Val& a, const Val& b) -> Val {
+        switch (op) {
+        case XG_RED_PROD: return convert(apply2<f_mul>(a, b), vdt);
+        case XG_RED_AMAX: return convert(apply2<f_maximum>(a, b), vdt);
+        case XG_RED_AMIN: return convert(apply2<f_minimum>(a, b), vdt);
+        case XG_RED_NORM_LINF: return truth(apply2<f_lt>(a, b)) ? b : a;
+        default: return convert(apply2<f_plus>(a, b), vdt);
+        }
+    };
+    reduce_loop<Val>(ev, in, ax, keep, init, r->has_initial != 0, ini, fn, mg, [&](int64_t o, Val v) {
+        if (op == XG_RED_MEAN) {
+            // mean_division: div = double(e.size() / s.size()), 0 when s is empty
+            int64_t esz = numel(in);
+            double div = n_out != 0 ? static_cast<double>(esz / n_out) : 0.0;
+            v = apply2<f_div>(v, make<double>(div));
+        } else if (op == XG_RED_NORM_L2) {
+            v = apply1<f_sqrt>(v);
+        }
+        store(*out, emit_offset(o), v);
+    });
+}
+
+// ---------------------------------------------------------------- scans
+int scan_result_dt(int op, int acc, int edt) {
+    int idt = acc >= 0 ? acc : edt;
+    return visit(idt, [&](auto ti) {
+        using IT = typename decltype(ti)::type;
+        return visit(edt, [&](auto te) {
+            using E = typename decltype(te)::type;
+            if (op == XG_SCAN_CUMSUM) return dt_of<decltype(IT() + E())>();
+            return dt_of<decltype(IT() * E())>();
+        });
+    });
+}
+
+void do_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, bool high_precision) {
+    Tree t = analyse(e);
+    Shape in = t.shape;
+    int nd = int(in.size());
+    int edt = t.node_dt[size_t(e->n_nodes - 1)];
+    int rdt = scan_result_dt(s->op, s->acc_dtype, edt);
+    bool flat = s->axis == XG_AXIS_NONE;
+    int axis = s->axis;
+    if (!flat) {
+        if (axis < 0) axis += nd;
+        if (axis < 0 || axis >= nd) fail(XG_BAD_AXES, "Axis larger than expression dimension in accumulator.");
+    }
+    Shape rs = flat ? Shape{numel(in)} : in;
+    Shape os(out->shape, out->shape + out->ndim);
+    if (os != rs) fail(XG_BROADCAST_ERROR, "output shape does not match scan shape");
+    int64_t total = numel(in);
+    if (total == 0) return;
+    // res = e (static_cast into the result type), row-major copy
+    Evaluator ev(e, in);
+    std::vector<Val> res(static_cast<size_t>(total));
+    std::vector<long double> hp;
+    if (high_precision) hp.resize(static_cast<size_t>(total));
+    std::vector<int64_t> idx(size_t(nd), 0);
+    int64_t p = 0;
+    do {
+        const Val& v = ev.at(idx.data(), nd);
+        if (high_precision) hp[size_t(p)] = (long double)cast_to<double>(v);
+        else res[size_t(p)] = convert(v, rdt);
+        ++p;
+    } while (nd > 0 && next_index(idx, in));
+    // loop sizes of accumulator_impl (row-major result)
+    int64_t inner_stride, outer_loop, inner_loop;
+    if (flat) {
+        inner_stride = 1; outer_loop = 1; inner_loop = total;
+    } else {
+        inner_stride = 1;
+        for (int k = axis + 1; k < nd; ++k) inner_stride *= in[size_t(k)];
+        int64_t init_size = in[size_t(axis)] != 1 ? 1 : 0;
+        outer_loop = init_size;
+        for (int k = 0; k < axis; ++k) outer_loop *= in[size_t(k)];
+        inner_loop = 1;
+        for (int k = axis; k < nd; ++k) inner_loop *= in[size_t(k)];
+    }
+    inner_loop -= inner_stride;
+    int64_t pos = 0;
+    for (int64_t i = 0; i < outer_loop; ++i) {
+        for (int64_t j = 0; j < inner_loop; ++j) {
+            size_t a = size_t(pos), b = size_t(pos + inner_stride);
+            if (high_precision) {
+                hp[b] = s->op == XG_SCAN_CUMSUM ? hp[a] + hp[b] : hp[a] * hp[b];
+            } else {
+                res[b] = convert(s->op == XG_SCAN_CUMSUM ? apply2<f_plus>(res[a], res[b]) : apply2<f_mul>(res[a], res[b]), rdt);
+            }
+            pos += 1;
+        }
+        pos += inner_stride;
+    }
+    // write out (row-major)
+    std::vector<int64_t> oidx(os.size(), 0);
+    int64_t q = 0;
+    do {
+        Val v = high_precision ? make<double>(double(hp[size_t(q)])) : res[size_t(q)];
+        store(*out, out_offset(*out, oidx.data()), v);
+        ++q;
+    } while (next_index(oidx, os));
+}
+
+template <class F> int guard(F&& f) {
+    try {
+        f();
+        g_err.clear();
+        return XG_OK;
+    } catch (const Fail& x) {
+        g_err = x.msg;
+        return x.code;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* xo_last_error(void) { return g_err.c_str(); }
+
+// shape and value type of an expression (oracle's own derivation)
+int xo_expr_info(const xg_expr* e, int32_t* dtype, int32_t* ndim, int64_t* shape) {
+    return guard([&] {
+        Tree t = analyse(e);
+        *dtype = t.node_dt[size_t(e->n_nodes - 1)];
+        *ndim = int32_t(t.shape.size());
+        for (size_t k = 0; k < t.shape.size(); ++k) shape[k] = t.shape[k];
+    });
+}
+
+int xo_assign(const xg_expr* e, const xg_tensor* out) {
+    return guard([&] { do_assign(e, out); });
+}
+
+int xo_reduce_info(const xg_expr* e, const xg_reduce_desc* r, int32_t* dtype, int32_t* ndim, int64_t* shape) {
+    return guard([&] {
+        Tree t = analyse(e);
+        std::vector<int> ax = normalise_axes(r, int(t.shape.size()));
+        Shape rs = reduce_shape(t.shape, ax, r->keep_dims != 0);
+        int rdt = reduce_result_dt(r->op, r->acc_dtype, t.node_dt[size_t(e->n_nodes - 1)]);
+        if (rdt < 0) fail(XG_TYPE_ERROR, "reduction not defined for this type");
+        *dtype = rdt;
+        *ndim = int32_t(rs.size());
+        for (size_t k = 0; k < rs.size(); ++k) shape[k] = rs[k];
+    });
+}
+
+// xtensor-order reduction (exact restatement)
+int xo_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out) {
+    return guard([&] { do_reduce(e, r, out, false); });
+}
+
+// same reduction accumulated in long double, written as float64 (error reference)
+int xo_reduce_hp(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out) {
+    return guard([&] { do_reduce(e, r, out, true); });
+}
+
+int xo_scan_info(const xg_expr* e, const xg_scan_desc* s, int32_t* dtype, int32_t* ndim, int64_t* shape) {
+    return guard([&] {
+        Tree t = analyse(e);
+        *dtype = scan_result_dt(s->op, s->acc_dtype, t.node_dt[size_t(e->n_nodes - 1)]);
+        if (s->axis == XG_AXIS_NONE) {
+            *ndim = 1;
+            shape[0] = numel(t.shape);
+        } else {
+            int nd = int(t.shape.size());
+            int a = s->axis < 0 ? s->axis + nd : s->axis;
+            if (a < 0 || a >= nd) fail(XG_BAD_AXES, "Axis larger than expression dimension in accumulator.");
+            *ndim = nd;
+            for (int k = 0; k < nd; ++k) shape[k] = t.shape[size_t(k)];
+        }
+    });
+}
+
+int xo_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out) {
+    return guard([&] { do_scan(e, s, out, false); });
+}
+
+int xo_scan_hp(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out) {
+    return guard([&] { do_scan(e, s, out, true); });
+}
+
+}  // extern "C"

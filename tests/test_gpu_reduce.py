@@ -1,0 +1,247 @@
+"""GPU parity of axis reductions (xg_reduce) against the oracle.
+
+Bar: integer reductions, amax / amin and norm_linf bit-exact (exact
+arithmetic, order independent); float sums / means / norms within a stated
+relative error of the long-double reference (the GPU tree order differs
+from xtensor's sequential fold by design, and is more accurate):
+  * float64 data: rel <= 1e-14
+  * float32 / bf16 data (float32 accumulation with compensation): rel <= 2e-7
+    on the float32 result (1 ulp is 6e-8 relative), 1e-12 for results kept
+    in double before rounding (mean, norms).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import xtensor_amd as xt
+from xtensor_amd import _abi as A
+
+from helpers import Dual, oracle_for, to_bf16_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def rel_err(got, ref):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    den = np.maximum(np.abs(ref), 1e-300)
+    return float(np.max(np.abs(got - ref) / den)) if ref.size else 0.0
+
+
+def run(r: xt.Reducer):
+    out = xt.eval(r)
+    sync()
+    return out.to_numpy(), xt.last_kernel()
+
+
+# ---------------------------------------------------------------- reference KATs on the device
+def test_kat_features_sum():
+    """test_xreducer.cpp:222-240, 355-362, 475-481."""
+    a = np.ones((3, 2, 4, 6, 5))
+    a[1, :, 1, :, 1] = 2
+    d = xt.from_numpy(a)
+    r, _ = run(xt.sum(d, [1, 3]))
+    exp = np.full((3, 4, 5), 12.0)
+    exp[1, 1, 1] = 24
+    assert np.array_equal(r, exp)
+    assert run(xt.sum(d))[0][()] == 732
+
+
+def test_kat_small_sums_and_means():
+    """test_xreducer.cpp:390-397, 442-473, 542-569; test_xmath.cpp:857-870."""
+    m = xt.from_numpy(np.array([[1.0, 2.0], [3.0, 4.0]]))
+    assert np.array_equal(run(xt.sum(m, [0]))[0], [4.0, 6.0])
+    u = xt.from_numpy(np.ones((2, 4)))
+    assert np.array_equal(run(xt.sum(u, [-1]))[0], [4.0, 4.0])
+    c = xt.from_numpy(np.ones(1000, np.uint8))
+    r, _ = run(xt.sum(c))
+    assert r.dtype == np.int32 and r[()] == 1000
+    x = xt.from_numpy(np.array([[-1.0, 0.0], [1.0, 0.0]]))
+    assert run(xt.mean(x))[0][()] == 0.0
+    assert np.array_equal(run(xt.mean(x, [1]))[0], [-0.5, 0.5])
+    c2 = xt.from_numpy(np.array([1, 2], np.uint8))
+    assert run(xt.mean(c2))[0][()] == 1.5
+    v = xt.from_numpy(np.array([[1.0, 1.0, 1.0], [2.0, 2.0, 2.0]]))
+    assert np.array_equal(run(xt.mean(v, [0]))[0], [1.5, 1.5, 1.5])
+    assert run(xt.mean(v, [0, 1]))[0][()] == 1.5
+
+
+def test_kat_keep_dims_initial_empty():
+    """test_xreducer.cpp:964-1079."""
+    a_h = np.arange(625, dtype=np.float64).reshape(5, 5, 5, 5)
+    a = xt.from_numpy(a_h)
+    r, _ = run(xt.sum(a, [0, 1], keep_dims=True))
+    assert r.shape == (1, 1, 5, 5) and np.array_equal(r[0, 0], a_h.sum(axis=(0, 1)))
+    base, _ = run(xt.sum(a, [0, 2], keep_dims=True))
+    assert np.array_equal(run(xt.sum(a, [0, 2], keep_dims=True, initial=5))[0], base + 5)
+    assert np.all(run(xt.prod(a, [1, 2], initial=0))[0] == 0)
+    x = xt.from_numpy(np.zeros((0, 1), np.int32))
+    r0, _ = run(xt.sum(x, [0], keep_dims=True))
+    assert r0.shape == (1, 1) and r0[0, 0] == 0
+    assert run(xt.sum(x, [1], keep_dims=True))[0].shape == (0, 1)
+    e = xt.from_numpy(np.ones((1, 2, 0, 1)))
+    assert math.isnan(run(xt.mean(e))[0][()])
+    r2, _ = run(xt.mean(e, [2]))
+    assert r2.shape == (1, 2, 1) and np.all(np.isnan(r2))
+    ones = xt.from_numpy(np.ones((1, 1, 2, 4), np.int32))
+    assert np.array_equal(run(xt.sum(ones, [1, 2], keep_dims=True))[0], [[[[2, 2, 2, 2]]]])
+    b = xt.from_numpy(np.array([[1, 2, 3], [4, 5, 6]], np.int32))
+    assert np.array_equal(run(xt.sum(b, []))[0], [[1, 2, 3], [4, 5, 6]])
+
+
+def test_kat_norms_and_extrema():
+    """test_xnorm.cpp:170-187, test_xmath.cpp:147-168."""
+    a = xt.from_numpy(np.array([[-1.0, 2.0], [-3.0, 4.0]]))
+    assert run(xt.norm_l0(a))[0][()] == 4
+    assert run(xt.norm_l1(a))[0][()] == 10.0
+    assert run(xt.norm_sq(a))[0][()] == 30.0
+    assert run(xt.norm_linf(a))[0][()] == 4.0
+    for vals, mn, mx in [([-10.0], -10.0, -10.0), ([-10.0, -20.0], -20.0, -10.0), ([-10.0, 20.0], -10.0, 20.0)]:
+        d = xt.from_numpy(np.array(vals))
+        assert run(xt.amin(d))[0][()] == mn and run(xt.amax(d))[0][()] == mx
+
+
+@pytest.mark.parametrize("axes", [[1, 0], [0, 2], [0, 0]])
+def test_bad_axes_raise(axes):
+    a = xt.from_numpy(np.ones((3, 2), np.int32))
+    with pytest.raises(RuntimeError):
+        xt.eval(xt.sum(a, axes))
+
+
+# ---------------------------------------------------------------- fused expression + reductions (f64)
+def test_fused_row_sum_and_column_amax_f64():
+    """s = sum(c, {1}), m = amax(c, {0}) for c = a*b + exp(-a*a) [2048, 1024] f64."""
+    rng = np.random.default_rng(0)
+    a_h = rng.uniform(-1, 1, (2048, 1024))
+    b_h = rng.uniform(-1, 1, (1024,))
+    a, b = xt.from_numpy(a_h), xt.from_numpy(b_h)
+    c = xt.eval(a * b + xt.exp(-a * a))
+    c_h = c.to_numpy()
+    C_ = Dual(c, c_h)
+    o = oracle_for(C_)
+    s, ks = run(xt.sum(c, [1]))
+    m, km = run(xt.amax(c, [0]))
+    assert rel_err(s, o.reduce(xt.sum(c, [1]))) <= 1e-14
+    assert rel_err(s, o.reduce(xt.sum(c, [1]), high_precision=True)) <= 1e-14
+    assert np.array_equal(m, o.reduce(xt.amax(c, [0])))
+    # fused: the expression evaluated inside the reduction
+    sf, _ = run(xt.sum(a * b + xt.exp(-a * a), [1]))
+    assert rel_err(sf, s) <= 1e-14
+    print("kernels", ks, km)
+
+
+# ---------------------------------------------------------------- float32 / bf16 / int32 matrix reductions
+def matrix(rows, cols, seed=2):
+    rng = np.random.default_rng(seed)
+    return rng.normal(1, 1, (rows, cols)).astype(np.float32)
+
+
+@pytest.mark.parametrize("rows,cols", [(512, 4096), (8192, 4096), (3, 1000), (1000, 3), (1, 1)])
+def test_f32_sum_mean_amax_norm(rows, cols):
+    x = matrix(rows, cols)
+    X = Dual.of(x)
+    o = oracle_for(X)
+    errs = {}
+    for ax in ([0], [1]):
+        for kind in ("sum", "mean"):
+            r = xt.Reducer(kind, X.dev, ax)
+            got, k = run(r)
+            ref = o.reduce(r, high_precision=True)
+            tol = 2e-7 if kind == "sum" else 2e-7
+            errs[(kind, ax[0], k)] = rel_err(got, ref)
+            assert errs[(kind, ax[0], k)] <= tol, (kind, ax, k)
+    got, _ = run(xt.amax(X.dev))
+    assert got[()] == x.max()
+    got, _ = run(xt.amax(X.dev, [0]))
+    assert np.array_equal(got, x.max(axis=0))
+    r = xt.norm_l2(X.dev)
+    got, k = run(r)
+    errs[("norm_l2", k)] = rel_err(got, o.reduce(r, high_precision=True))
+    assert errs[("norm_l2", k)] <= 1e-12 or errs[("norm_l2", k)] <= 2e-7 * 2
+    print(errs)
+
+
+def test_f32_transposed_view():
+    x = matrix(1024, 2048, seed=5)
+    X = Dual.of(x)
+    XT = X.transpose()
+    o = oracle_for(XT)
+    for ax in ([0], [1]):
+        r = xt.sum(XT.dev, ax)
+        got, k = run(r)
+        assert rel_err(got, o.reduce(r, high_precision=True)) <= 2e-7, (ax, k)
+
+
+def test_bf16_reductions():
+    x = matrix(512, 4096, seed=7)
+    X = Dual.of(x, "bfloat16")
+    o = oracle_for(X)
+    for r in (xt.sum(X.dev, [0]), xt.sum(X.dev, [1]), xt.mean(X.dev, [1]), xt.norm_l2(X.dev)):
+        got, k = run(r)
+        assert rel_err(got, o.reduce(r, high_precision=True)) <= 2e-7, k
+    got, _ = run(xt.amax(X.dev))
+    assert got[()] == o.reduce(xt.amax(X.dev))[()]
+
+
+@pytest.mark.parametrize("shape", [(8192, 4096), (257, 3001)])
+def test_int32_sum_into_int64_exact(shape):
+    rng = np.random.default_rng(2)
+    x = rng.integers(-1000, 1001, shape, dtype=np.int32)
+    X = Dual.of(x)
+    for ax in ([0], [1]):
+        got, k = run(xt.sum(X.dev, ax, dtype="int64"))
+        assert got.dtype == np.int64
+        assert np.array_equal(got, x.astype(np.int64).sum(axis=ax[0])), k
+
+
+def test_int32_sum_wraps_like_cpp():
+    """sum(int32) has value type int: the total wraps modulo 2^32 exactly as xtensor's does."""
+    x = np.full((4, 1 << 16), 1 << 20, np.int32)
+    X = Dual.of(x)
+    got, _ = run(xt.sum(X.dev, [1]))
+    assert np.array_equal(got, oracle_for(X).reduce(xt.sum(X.dev, [1])))
+
+
+# ---------------------------------------------------------------- fuzz over shapes / axes / kinds / dtypes
+KINDS = ["sum", "prod", "amax", "amin", "mean", "norm_sq", "norm_l1", "norm_linf", "norm_l0", "norm_l2"]
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_reductions(seed):
+    rng = np.random.default_rng(500 + seed)
+    for _ in range(12):
+        nd = int(rng.integers(1, 5))
+        shape = tuple(int(rng.integers(1, 9)) for _ in range(nd))
+        dt = ["int32", "int64", "uint8", "float32", "float64", "int16"][rng.integers(6)]
+        kind = KINDS[rng.integers(len(KINDS))]
+        if dt.startswith("float"):
+            v = rng.uniform(0.5, 2.0, shape).astype(dt)
+        else:
+            v = rng.integers(-3 if dt != "uint8" else 0, 4, shape).astype(dt)
+        X = Dual.of(v)
+        # views: maybe transpose or reverse
+        if nd >= 2 and rng.random() < 0.3:
+            X = X.transpose()
+        axes = sorted(set(int(a) for a in rng.integers(0, nd, size=rng.integers(0, nd + 1))))
+        if rng.random() < 0.2:
+            axes = None
+        r = xt.Reducer(kind, X.dev, axes, keep_dims=bool(rng.random() < 0.3))
+        got, k = run(r)
+        o = oracle_for(X)
+        ref = o.reduce(r)
+        assert got.shape == ref.shape and got.dtype == ref.dtype
+        if ref.dtype.kind in "iub" or kind in ("amax", "amin", "norm_linf"):
+            assert np.array_equal(got, ref), (kind, dt, shape, axes, k)
+        else:
+            hp = o.reduce(r, high_precision=True)
+            tol = 1e-13 if ref.dtype == np.float64 and dt == "float64" else 3e-7
+            if kind == "prod":
+                tol = 1e-5 if dt == "float32" else 1e-12
+            assert rel_err(got, hp) <= tol, (kind, dt, shape, axes, k, rel_err(got, hp))

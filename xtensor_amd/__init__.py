@@ -1,0 +1,21 @@
+"""xtensor_amd — MI355X (gfx950) evaluator for xtensor's lazy expressions.
+
+`import xtensor_amd as xt` gives xtensor's names over device arrays:
+`xt.noalias(out).assign(a * b + xt.exp(-a * a))`, `xt.sum(e, [1])`,
+`xt.cumsum(e, 1)`, `xt.view(y, xt.all_(), xt.range_(0, 56, 2))` ...
+Every evaluation runs hand-written HIP kernels in `lib/libxg.so` through the C
+ABI of `include/xg.h`; there is no CPU fallback.
+"""
+
+from .core import (  # noqa: F401
+    BroadcastError, DeviceArray, DeviceError, Expr, Func, Reducer, Scalar, assign_checked,
+    abs, acos, all_, amax, amin, asarray, asin, assign, atan, atan2, broadcast, cast, cbrt, ceil, clip,
+    cos, cosh, cumprod, cumsum, dtype_code, dtype_name, empty, equal, erf, eval, exp, exp2, expm1,
+    expr_info, fabs, floor, fmod, from_numpy, greater, hypot, last_kernel, less, log, log1p, log2,
+    log10, logical_and, logical_not, logical_or, maximum, mean, minimum, newaxis, noalias,
+    norm_l0, norm_l1, norm_l2, norm_linf, norm_sq, not_equal, numpy_dtype, plan, Plan, pow, prod, range_,
+    reduce_info, reshape_view, round, scalar, serialize, sin, sinh, sqrt, strided_view, sum, tan,
+    tanh, transpose, trunc, view, where, zeros,
+)
+
+__version__ = "0.1.0"

@@ -1,0 +1,352 @@
+// api.cpp — the extern "C" entry points declared in include/xg.h.
+//
+// Each entry point validates the descriptor, plans on the host and enqueues
+// kernels on the caller's stream.  Errors become status codes with the
+// meaning of the exceptions xtensor throws at the same point
+// (broadcast_error, utils/xexception.hpp:63; std::runtime_error for axes,
+// reducers/xreducer.hpp:1426-1440 and reducers/xaccumulator.hpp:231-234).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include <map>
+
+#include "ew.hpp"
+#include "planner.hpp"
+#include "reduce.hpp"
+#include "scan.hpp"
+#include "xg_api_internal.hpp"
+
+namespace xg {
+
+namespace {
+thread_local std::string t_err;
+thread_local std::string t_last_kernel;
+
+struct DeviceInfo {
+    int cus = 0;
+    bool ok = false;
+};
+
+std::mutex g_dev_mu;
+std::vector<DeviceInfo> g_devs;
+}  // namespace
+
+void set_error(const std::string& m) { t_err = m; }
+
+int device_cus() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) raise(XG_DEVICE_ERROR, "hipGetDevice failed");
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (int(g_devs.size()) <= dev) g_devs.resize(size_t(dev) + 1);
+    DeviceInfo& d = g_devs[size_t(dev)];
+    if (!d.ok) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) != hipSuccess) raise(XG_DEVICE_ERROR, "hipGetDeviceProperties failed");
+        d.cus = prop.multiProcessorCount;
+        d.ok = true;
+    }
+    return d.cus;
+}
+
+void note_kernel(const char* name) { t_last_kernel = name ? name : ""; }
+
+void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) raise(XG_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+// Workspace (partial results of split reductions, scan temporaries): one
+// growing buffer per (device, stream), so work on one stream reuses it in
+// stream order and different streams never share one.
+struct WsKey {
+    int dev;
+    void* stream;
+    bool operator<(const WsKey& o) const { return dev != o.dev ? dev < o.dev : stream < o.stream; }
+};
+struct WsBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_ws_mu;
+std::map<WsKey, WsBuf> g_ws;
+}  // namespace
+
+void* workspace(size_t bytes, hipStream_t stream) {
+    if (bytes == 0) return nullptr;
+    int dev = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    WsBuf& b = g_ws[WsKey{dev, reinterpret_cast<void*>(stream)}];
+    if (b.bytes < bytes) {
+        if (b.ptr) {
+            // earlier work on this stream may still read the old buffer
+            check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            check_hip(hipFree(b.ptr), "hipFree");
+            b.ptr = nullptr;
+            b.bytes = 0;
+        }
+        size_t n = std::max<size_t>(bytes, size_t(1) << 20);
+        if (hipMalloc(&b.ptr, n) != hipSuccess) raise(XG_OUT_OF_MEMORY, "workspace allocation failed");
+        b.bytes = n;
+    }
+    return b.ptr;
+}
+
+}  // namespace xg
+
+using namespace xg;
+
+extern "C" {
+
+const char* xg_version(void) { return "xtensor_amd 0.1.0 (gfx950)"; }
+
+const char* xg_last_error_message(void) { return t_err.c_str(); }
+
+// Name of the kernel family the calling thread's last successful entry point
+// used ("dynamic", "spec:<name>", "reduce_rows", ...).  Diagnostic only.
+const char* xg_last_kernel(void) { return t_last_kernel.c_str(); }
+
+xg_status xg_init(void) {
+    return guard([&] { device_cus(); });
+}
+
+xg_status xg_expr_info(const xg_expr* e, int32_t* dtype, int32_t* ndim, int64_t* shape) {
+    return guard([&] {
+        if (!dtype || !ndim || !shape) raise(XG_INVALID_ARGUMENT, "null output pointer");
+        ExprInfo info = analyse(e);
+        *dtype = info.dtype;
+        *ndim = int32_t(info.shape.size());
+        for (size_t k = 0; k < info.shape.size(); ++k) shape[k] = info.shape[k];
+    });
+}
+
+xg_status xg_assign(const xg_expr* e, const xg_tensor* out, xg_stream_t stream) {
+    return guard([&] {
+        int V = choose_vector_width(e, out);
+        EwPlan p = plan_assign(e, out, V);
+        if (p.units == 0) {
+            note_kernel("empty");
+            return;
+        }
+        const char* used = nullptr;
+        check_hip(launch_ew(p, V, device_cus(), reinterpret_cast<hipStream_t>(stream), &used), "elementwise launch");
+        note_kernel(used);
+    });
+}
+
+xg_status xg_reduce_info(const xg_expr* e, const xg_reduce_desc* r, int32_t* dtype, int32_t* ndim, int64_t* shape) {
+    return guard([&] {
+        if (!dtype || !ndim || !shape) raise(XG_INVALID_ARGUMENT, "null output pointer");
+        int dt = 0;
+        Shape rs;
+        reduce_result(e, r, &dt, &rs);
+        *dtype = dt;
+        *ndim = int32_t(rs.size());
+        for (size_t k = 0; k < rs.size(); ++k) shape[k] = rs[k];
+    });
+}
+
+xg_status xg_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, xg_stream_t stream) {
+    return guard([&] {
+        hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+        int cus = device_cus();
+        int V = choose_vector_width(e, nullptr);
+        RedPlan p = plan_reduce(e, r, out, V, cus, nullptr, nullptr);
+        if (V != 4 && !find_rspec(p, V)) {  // the generic kernels are built for V = 4
+            V = 4;
+            p = plan_reduce(e, r, out, V, cus, nullptr, nullptr);
+        }
+        if (p.n_out == 0) {
+            note_kernel("empty");
+            return;
+        }
+        p.partials = workspace(reduce_workspace_bytes(p), st);
+        const char* used = nullptr;
+        check_hip(launch_reduce(p, V, st, &used), "reduction launch");
+        note_kernel(used);
+    });
+}
+
+xg_status xg_scan_info(const xg_expr* e, const xg_scan_desc* s, int32_t* dtype, int32_t* ndim, int64_t* shape) {
+    return guard([&] {
+        if (!dtype || !ndim || !shape) raise(XG_INVALID_ARGUMENT, "null output pointer");
+        int dt = 0;
+        Shape rs;
+        scan_result(e, s, &dt, &rs);
+        *dtype = dt;
+        *ndim = int32_t(rs.size());
+        for (size_t k = 0; k < rs.size(); ++k) shape[k] = rs[k];
+    });
+}
+
+xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, xg_stream_t stream) {
+    return guard([&] {
+        if (!s || !out) raise(XG_INVALID_ARGUMENT, "null argument");
+        hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+        int dt = 0;
+        Shape rs;
+        scan_result(e, s, &dt, &rs);
+        const int V = 4;
+        if (s->axis != XG_AXIS_NONE) {
+            ScanPlan p = plan_scan(e, s, out, V);
+            const char* used = nullptr;
+            check_hip(launch_scan(p, V, st, &used), "scan launch");
+            note_kernel(used);
+            return;
+        }
+        // flattened scan: evaluate e row-major into a contiguous temporary of
+        // the result type, then scan it as one row
+        if (out->ndim != 1 || out->shape[0] != rs[0]) raise(XG_BROADCAST_ERROR, "flattened scan output must be 1-d");
+        ExprInfo info = analyse(e);
+        int64_t n = numel(info.shape);
+        if (n == 0) return;
+        xg_tensor tmp{};
+        tmp.dtype = dt;
+        tmp.ndim = int32_t(info.shape.size());
+        int64_t stv = 1;
+        for (int k = tmp.ndim - 1; k >= 0; --k) {
+            tmp.shape[k] = info.shape[size_t(k)];
+            tmp.strides[k] = stv;
+            stv *= info.shape[size_t(k)];
+        }
+        tmp.data = workspace(size_t(n) * dtype_size(dt), st);
+        check_hip(hipSuccess, "");
+        {
+            int Vw = choose_vector_width(e, &tmp);
+            EwPlan ep = plan_assign(e, &tmp, Vw);
+            check_hip(launch_ew(ep, Vw, device_cus(), st, nullptr), "scan copy launch");
+        }
+        // 1-d view of the temporary as the only leaf
+        xg_tensor flat{};
+        flat.data = tmp.data;
+        flat.dtype = dt;
+        flat.ndim = 1;
+        flat.shape[0] = n;
+        flat.strides[0] = 1;
+        xg_node node{};
+        node.op = XG_OP_LEAF;
+        xg_expr fe{};
+        fe.n_nodes = 1;
+        fe.n_leaves = 1;
+        fe.nodes = &node;
+        fe.leaves = &flat;
+        xg_scan_desc s1 = *s;
+        s1.axis = 0;
+        ScanPlan p = plan_scan(&fe, &s1, out, V);
+        const char* used = nullptr;
+        check_hip(launch_scan(p, V, st, &used), "scan launch");
+        note_kernel(used);
+    });
+}
+
+// ---------------------------------------------------------------- plans
+struct xg_plan_s {
+    int kind = 0;  // 0 assign, 1 reduce, 2 scan
+    int V = 4;
+    xg::EwPlan ew;
+    xg::RedPlan red;
+    xg::ScanPlan scan;
+    size_t ws = 0;
+    std::string kernel;
+};
+
+xg_status xg_plan_assign(const xg_expr* e, const xg_tensor* out, xg_plan_t* plan) {
+    return guard([&] {
+        if (!plan) raise(XG_INVALID_ARGUMENT, "null plan pointer");
+        auto* p = new xg_plan_s();
+        p->kind = 0;
+        p->V = choose_vector_width(e, out);
+        try {
+            p->ew = plan_assign(e, out, p->V);
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *plan = p;
+    });
+}
+
+xg_status xg_plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, xg_plan_t* plan) {
+    return guard([&] {
+        if (!plan) raise(XG_INVALID_ARGUMENT, "null plan pointer");
+        auto* p = new xg_plan_s();
+        try {
+            p->kind = 1;
+            int cus = device_cus();
+            p->V = choose_vector_width(e, nullptr);
+            p->red = plan_reduce(e, r, out, p->V, cus, nullptr, nullptr);
+            if (p->V != 4 && !find_rspec(p->red, p->V)) {
+                p->V = 4;
+                p->red = plan_reduce(e, r, out, p->V, cus, nullptr, nullptr);
+            }
+            p->ws = reduce_workspace_bytes(p->red);
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *plan = p;
+    });
+}
+
+xg_status xg_plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, xg_plan_t* plan) {
+    return guard([&] {
+        if (!plan || !s) raise(XG_INVALID_ARGUMENT, "null argument");
+        if (s->axis == XG_AXIS_NONE) raise(XG_UNSUPPORTED, "flattened scans are not planned; use xg_scan");
+        auto* p = new xg_plan_s();
+        try {
+            p->kind = 2;
+            p->V = 4;
+            p->scan = plan_scan(e, s, out, p->V);
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *plan = p;
+    });
+}
+
+size_t xg_plan_workspace_size(xg_plan_t plan) { return plan ? plan->ws : 0; }
+
+xg_status xg_plan_execute(xg_plan_t p, void* ws, xg_stream_t stream) {
+    return guard([&] {
+        if (!p) raise(XG_INVALID_ARGUMENT, "null plan");
+        hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+        const char* used = nullptr;
+        if (p->kind == 0) {
+            if (p->ew.units) check_hip(launch_ew(p->ew, p->V, device_cus(), st, &used), "elementwise launch");
+        } else if (p->kind == 1) {
+            if (p->red.n_out) {
+                p->red.partials = p->ws ? (ws ? ws : workspace(p->ws, st)) : nullptr;
+                check_hip(launch_reduce(p->red, p->V, st, &used), "reduction launch");
+            }
+        } else {
+            check_hip(launch_scan(p->scan, p->V, st, &used), "scan launch");
+        }
+        note_kernel(used);
+    });
+}
+
+void xg_plan_destroy(xg_plan_t plan) { delete plan; }
+
+// Testing aid: 1 routes every assignment through the dynamic interpreter.
+void xg_debug_force_dynamic(int on) { ew_set_force_dynamic(on); }
+
+// Name of the compiled specialisation an assignment would use, "" if none.
+xg_status xg_debug_assign_kernel(const xg_expr* e, const xg_tensor* out, char* name, int32_t cap) {
+    return guard([&] {
+        int V = choose_vector_width(e, out);
+        EwPlan p = plan_assign(e, out, V);
+        const char* s = ew_find_spec(p, V);
+        std::string n = s ? s : "";
+        if (cap > 0) {
+            std::strncpy(name, n.c_str(), size_t(cap - 1));
+            name[cap - 1] = 0;
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+// reduce.hip — axis reductions (sum, prod, amax, amin, mean, norms) on gfx950.
+//
+// Replaces xreducer's sequential per-output fold
+// (reducers/xreducer.hpp:1801-1828) by a tree: every thread folds a strided
+// subset of an output's elements, then threads combine with wave shuffles
+// and LDS in a fixed order, so a result depends only on the shape of its
+// reduction.  Two layouts cover any axis set:
+//   rows: the contiguous dim is reduced; a group of 1..256 threads per
+//         output reads its elements with vector loads;
+//   cols: the contiguous dim is kept; each thread owns V adjacent outputs
+//         and walks the reduced rows, four waves splitting the rows.
+// Large reductions are split over several workgroups; their partial
+// accumulators are combined by a second kernel in split order.
+// Float sums carry a TwoSum compensation term (Neumaier / Kahan-Babuska
+// summation with error-free pairwise combining); integer reductions use the
+// exact two's-complement arithmetic of the value type.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "reduce_kernels.hpp"
+
+
+#include "specs.inc"
+
+namespace xg {
+
+struct RSpecEntry {
+    const char* name;
+    int V, acc, op;
+    SpecKey key;
+    RedKernels k;
+};
+
+namespace {
+
+template <class S> SpecKey rkey_of() {
+    SpecKey k{};
+    k.nops = S::nops;
+    for (int i = 0; i < S::nops; ++i) k.ops[i] = S::ops[i];
+    k.nleaves = S::nleaves;
+    for (int l = 0; l < S::nleaves; ++l) {
+        k.leaf_dtype[l] = S::leaf_dtype[l];
+        k.leaf_mode[l] = S::leaf_mode[l];
+    }
+    k.result_reg = S::result_reg;
+    return k;
+}
+
+std::vector<RSpecEntry>& rspec_table() {
+    static std::vector<RSpecEntry> t;
+    static std::once_flag once;
+    std::call_once(once, [] {
+#define XG_REGISTER_RSPEC(S) \
+    t.push_back(RSpecEntry{#S, S::V, S::acc, S::op, rkey_of<S>(), spec_kernels_for<S::V, S::nregs, S, S::acc, S::op>()});
+        XG_RSPEC_LIST(XG_REGISTER_RSPEC)
+#undef XG_REGISTER_RSPEC
+    });
+    return t;
+}
+
+}  // namespace
+
+// a compiled specialisation for (program, V, accumulator, op), or nullptr
+const RSpecEntry* find_rspec(const RedPlan& p, int V) {
+    SpecKey k{};
+    k.nops = p.prog.nops;
+    for (int i = 0; i < p.prog.nops; ++i) k.ops[i] = p.prog.ops[i];
+    k.nleaves = p.prog.nleaves;
+    for (int l = 0; l < p.prog.nleaves; ++l) {
+        k.leaf_dtype[l] = uint8_t(p.leaves[l].dtype);
+        k.leaf_mode[l] = uint8_t(p.leaves[l].mode);
+    }
+    k.result_reg = p.prog.result_reg;
+    for (auto& s : rspec_table())
+        if (s.V == V && s.acc == p.acc && s.op == p.op && spec_equal(s.key, k)) return &s;
+    return nullptr;
+}
+
+hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** used) {
+    if (p.n_out == 0) return hipSuccess;
+    RedKernels ks{};
+    const RSpecEntry* sp = p.index64 ? nullptr : find_rspec(p, V);
+    if (sp) {
+        ks = sp->k;
+        if (used) *used = sp->name;
+    } else {
+        if (V != 4) return hipErrorInvalidValue;  // dynamic kernels are built for V = 4
+        if (!dyn_reduce_float(p.acc, p.op, &ks) && !dyn_reduce_int32(p.acc, p.op, &ks) &&
+            !dyn_reduce_int64(p.acc, p.op, &ks))
+            return hipErrorInvalidValue;
+        if (used) *used = "dynamic";
+    }
+    dim3 block(256);
+    if (p.layout == RL_ROWS) {
+        uint64_t items = uint64_t(p.n_out) * p.splits;
+        uint64_t per_block = 256 / p.group;
+        uint64_t blocks = (items + per_block - 1) / per_block;
+        blocks = std::min<uint64_t>(std::max<uint64_t>(blocks, 1), 1u << 20);
+        KFn k = p.index64 ? ks.rows64 : ks.rows32;
+        if (!k) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k, dim3(uint32_t(blocks)), block, 0, stream, p);
+    } else {
+        uint64_t items = uint64_t(p.n_out / p.lf) * p.col_tiles * p.splits;
+        uint64_t blocks = std::min<uint64_t>(std::max<uint64_t>(items, 1), 1u << 20);
+        KFn k = p.index64 ? ks.cols64 : ks.cols32;
+        if (!k) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k, dim3(uint32_t(blocks)), block, 0, stream, p);
+    }
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess || p.splits == 1) return err;
+    KFn comb = combine_kernel(p.acc, p.op);
+    if (!comb) return hipErrorInvalidValue;
+    uint64_t blocks = std::min<uint64_t>((p.n_out + 255) / 256, 1u << 20);
+    hipLaunchKernelGGL(comb, dim3(uint32_t(std::max<uint64_t>(blocks, 1))), block, 0, stream, p);
+    return hipGetLastError();
+}
+
+}  // namespace xg

@@ -1,0 +1,87 @@
+// reduce.hpp — plans and host entry points of the axis-reduction kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "planner.hpp"
+#include "xg_internal.hpp"
+
+namespace xg {
+
+enum RedLayout : int32_t {
+    RL_ROWS = 0,  // the contiguous (fast) dim is reduced: a thread group per output
+    RL_COLS = 1   // the fast dim is kept: a thread per V adjacent outputs, rows split over waves
+};
+
+// Accumulation type of the reduction tree.
+enum AccKind : int32_t {
+    AK_F32C = 0,  // float sum with a TwoSum (Neumaier / Kahan-Babuska) compensation term
+    AK_F64C,      // double sum with compensation
+    AK_F32,       // float, for products, maxima, minima
+    AK_F64,
+    AK_I32,       // integers: exact two's-complement arithmetic of the value type
+    AK_U32,
+    AK_I64,
+    AK_U64,
+    AK_NUM
+};
+
+enum RedOp : int32_t { RO_ADD = 0, RO_MUL, RO_MAX, RO_MIN, RO_NUM };
+
+enum Finalize : int32_t { FZ_NONE = 0, FZ_MEAN = 1, FZ_SQRT = 2 };
+
+struct RedPlan {
+    int32_t layout;
+    int32_t rank;       // normalised dims (see below)
+    int32_t acc;        // AccKind
+    int32_t op;         // RedOp
+    int32_t fin;        // Finalize
+    int32_t index64;
+    // RL_ROWS: dims = [kept 0..n_kept-1 | reduced outer | reduced fast]
+    // RL_COLS: dims = [kept outer 0..n_kept-1 | reduced ... | kept fast]
+    int32_t n_kept;     // leading kept dims
+    int32_t n_red;      // reduced dims
+    int64_t shape[kMaxRank];
+    FastDiv div[kMaxRank];
+    uint32_t n_out;     // number of output elements
+    uint32_t lf;        // length of the last (fast) dim
+    uint32_t nchunks_f; // ceil(lf / V)
+    FastDiv div_nchunks_f;
+    uint32_t red_count; // reduced elements per output (RL_COLS: rows; RL_ROWS: chunks)
+    int32_t group;      // RL_ROWS threads per output (1..256, power of two)
+    int32_t splits;     // partial results per output (1 = write the result directly)
+    uint32_t per_split; // RL_ROWS chunks / RL_COLS rows handled by one split
+    uint32_t col_tiles; // RL_COLS: ceil(nchunks_f / 64)
+    FastDiv div_col_tiles;
+    FastDiv div_splits;
+    // finishing: acc -> value type V (vdt) -> [+initial] -> mean / sqrt (rdt) -> out dtype
+    int32_t vdt;
+    int32_t rdt;
+    int32_t has_initial;
+    int32_t out_dtype;
+    uint64_t identity;  // start value of every accumulator, in acc representation
+    uint64_t initial;   // xt::initial, in acc representation
+    double mean_div;
+    char* out_ptr;
+    int64_t out_stride[kMaxRank];  // over the kept dims in normalised order (fast last for RL_COLS)
+    void* partials;     // workspace: splits * n_out accumulators of 16 bytes
+    LeafView leaves[kMaxLeaves];
+    Program prog;       // element value mapped and converted to the accumulation type
+};
+
+// Plan out = reduce(e).  Throws Error.  `V` is the vector width.
+RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, int V, int cus,
+                    int* result_dtype, Shape* result_shape);
+
+// Result dtype / shape only.
+void reduce_result(const xg_expr* e, const xg_reduce_desc* r, int* dtype, Shape* shape);
+
+hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** used);
+
+size_t reduce_workspace_bytes(const RedPlan& p);
+
+struct RSpecEntry;
+// compiled specialisation matching the plan, or nullptr
+const RSpecEntry* find_rspec(const RedPlan& p, int V);
+
+}  // namespace xg

@@ -1,0 +1,285 @@
+// reduce_kernels.hpp — the reduction kernel templates, included by the
+// translation units that instantiate them (split so they compile in
+// parallel).  See reduce.hip for the design.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "acc.hpp"
+#include "chunk.hpp"
+#include "reduce.hpp"
+
+namespace xg {
+
+// ---------------------------------------------------------------- finishing
+// output element o (row-major over the kept dims) -> output element offset
+__device__ __forceinline__ int64_t out_offset(const RedPlan& p, uint32_t o) {
+    int64_t off = 0;
+    if (p.layout == RL_COLS) {
+        uint32_t q = fastdiv(o, p.div[p.rank - 1]);
+        off += int64_t(o - q * p.lf) * p.out_stride[p.rank - 1];
+        o = q;
+    }
+    for (int k = p.n_kept - 1; k >= 0; --k) {
+        uint32_t q = fastdiv(o, p.div[k]);
+        off += int64_t(o - q * uint32_t(p.shape[k])) * p.out_stride[k];
+        o = q;
+    }
+    return off;
+}
+
+template <class A> __device__ void finish(const RedPlan& p, A acc, uint32_t o) {
+    if (p.has_initial) {  // xreducer merges `initial` in front of the folded value
+        A ini;
+        ini.init(p.initial);
+        ini.merge(acc);
+        acc = ini;
+    }
+    Reg v = acc.value(p.vdt);
+    Reg r = v;
+    if (p.fin == FZ_MEAN) {
+        double d = as_f64(cvt(v, p.vdt, XG_FLOAT64));
+        r = from_f64(d / p.mean_div);
+    } else if (p.fin == FZ_SQRT) {
+        if (p.rdt == XG_FLOAT32) r = from_f32(sqrtf(as_f32(cvt(v, p.vdt, XG_FLOAT32))));
+        else r = from_f64(sqrt(as_f64(cvt(v, p.vdt, XG_FLOAT64))));
+    }
+    const int odt = p.out_dtype;
+    Reg ov = odt == XG_BFLOAT16 ? cvt(r, p.rdt, XG_FLOAT32) : cvt(r, p.rdt, odt);
+    const int esz = dtype_size(odt);
+    store_raw(p.out_ptr + out_offset(p, o) * esz, esz, to_storage(ov, odt));
+}
+
+// group-wide combine of G (power of two) threads; all threads of the block call it
+template <class A>
+__device__ A group_combine(A a, int G, Store* lds) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wg = G < 64 ? G : 64;
+    for (int m = 1; m < wg; m <<= 1) {
+        A o = a.shfl_xor(m);
+        if (lane & m) {  // combine in index order: lower lane's value first
+            o.merge(a);
+            a = o;
+        } else {
+            a.merge(o);
+        }
+    }
+    if (G > 64) {
+        if (lane == 0) lds[wave] = a.store();
+        __syncthreads();
+        const int per = G / 64;
+        const int first = (wave / per) * per;
+        A r;
+        r.load(lds[first]);
+        for (int w = 1; w < per; ++w) {
+            A o;
+            o.load(lds[first + w]);
+            r.merge(o);
+        }
+        __syncthreads();
+        a = r;
+    }
+    return a;
+}
+
+
+// ---------------------------------------------------------------- rows layout
+template <int V, int NR, class S, bool I64, int AK, int OP, int kUnroll>
+__global__ __launch_bounds__(256) void reduce_rows_kernel(const RedPlan p) {
+    using A = Acc<AK, OP>;
+    __shared__ Store lds[4];
+    const int G = p.group;
+    const int per_block = 256 / G;
+    const uint32_t grp = threadIdx.x / G;
+    const uint32_t gl = threadIdx.x % G;
+    const uint32_t items = p.n_out * uint32_t(p.splits);
+    for (uint32_t base = blockIdx.x * per_block; base < items; base += gridDim.x * per_block) {
+        const uint32_t wi = base + grp;
+        const bool active = wi < items;
+        A acc;
+        acc.init(p.identity);
+        uint32_t o = 0;
+        if (active) {
+            o = fastdiv(wi, p.div_splits);
+            const uint32_t sp = wi - o * uint32_t(p.splits);
+            uint32_t idx[kMaxRank];
+#pragma unroll
+            for (int k = 0; k < kMaxRank; ++k) idx[k] = 0;
+            uint32_t q = o;
+            for (int k = p.n_kept - 1; k >= 0; --k) {
+                uint32_t d = fastdiv(q, p.div[k]);
+                idx[k] = q - d * uint32_t(p.shape[k]);
+                q = d;
+            }
+            const uint32_t c0 = sp * p.per_split;
+            const uint32_t c1 = min(p.red_count, c0 + p.per_split);
+            for (uint32_t c = c0 + gl; c < c1; c += G * kUnroll) {
+                Reg vals[kUnroll][V];
+                int nv[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    const uint32_t cc = c + u * G;
+                    nv[u] = 0;
+                    if (cc < c1) {
+                        uint32_t ro = fastdiv(cc, p.div_nchunks_f);
+                        const uint32_t j0 = (cc - ro * p.nchunks_f) * V;
+                        uint32_t id2[kMaxRank];
+#pragma unroll
+                        for (int k = 0; k < kMaxRank; ++k) id2[k] = idx[k];
+                        for (int k = p.rank - 2; k >= p.n_kept; --k) {
+                            uint32_t d = fastdiv(ro, p.div[k]);
+                            id2[k] = ro - d * uint32_t(p.shape[k]);
+                            ro = d;
+                        }
+                        nv[u] = min(V, int(p.lf - j0));
+                        eval_chunk<V, NR, S, I64>(p.leaves, p.prog, p.rank, id2, j0, nv[u], vals[u]);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+                    for (int v = 0; v < V; ++v)
+                        if (v < nv[u]) acc.add(vals[u][v]);
+            }
+        }
+        acc = group_combine(acc, G, lds);
+        if (active && gl == 0) {
+            if (p.splits == 1) finish(p, acc, o);
+            else reinterpret_cast<Store*>(p.partials)[wi] = acc.store();
+        }
+    }
+}
+
+// ---------------------------------------------------------------- cols layout
+template <int V, int NR, class S, bool I64, int AK, int OP, int kUnroll>
+__global__ __launch_bounds__(256) void reduce_cols_kernel(const RedPlan p) {
+    using A = Acc<AK, OP>;
+    __shared__ Store lds[4][64][V];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const uint32_t n_ko = p.n_out / p.lf;
+    const uint32_t items = n_ko * p.col_tiles * uint32_t(p.splits);
+    for (uint32_t wi = blockIdx.x; wi < items; wi += gridDim.x) {
+        const uint32_t rest = fastdiv(wi, p.div_splits);
+        const uint32_t sp = wi - rest * uint32_t(p.splits);
+        const uint32_t ko = fastdiv(rest, p.div_col_tiles);
+        const uint32_t tile = rest - ko * p.col_tiles;
+        const uint32_t cc = tile * 64 + lane;
+        const bool active = cc < p.nchunks_f;
+        const uint32_t j0 = cc * V;
+        const int nvalid = active ? min(V, int(p.lf - j0)) : 0;
+        uint32_t idx[kMaxRank];
+#pragma unroll
+        for (int k = 0; k < kMaxRank; ++k) idx[k] = 0;
+        uint32_t q = ko;
+        for (int k = p.n_kept - 1; k >= 0; --k) {
+            uint32_t d = fastdiv(q, p.div[k]);
+            idx[k] = q - d * uint32_t(p.shape[k]);
+            q = d;
+        }
+        A acc[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v].init(p.identity);
+        if (active) {
+            const uint32_t r0 = sp * p.per_split;
+            const uint32_t r1 = min(p.red_count, r0 + p.per_split);
+            for (uint32_t r = r0 + wave; r < r1; r += 4 * kUnroll) {
+                Reg vals[kUnroll][V];
+                bool ok[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    const uint32_t rr = r + 4 * u;
+                    ok[u] = rr < r1;
+                    if (ok[u]) {
+                        uint32_t id2[kMaxRank];
+#pragma unroll
+                        for (int k = 0; k < kMaxRank; ++k) id2[k] = idx[k];
+                        uint32_t t = rr;
+                        for (int k = p.rank - 2; k >= p.n_kept; --k) {
+                            uint32_t d = fastdiv(t, p.div[k]);
+                            id2[k] = t - d * uint32_t(p.shape[k]);
+                            t = d;
+                        }
+                        eval_chunk<V, NR, S, I64>(p.leaves, p.prog, p.rank, id2, j0, nvalid, vals[u]);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u)
+                    if (ok[u])
+#pragma unroll
+                        for (int v = 0; v < V; ++v) acc[v].add(vals[u][v]);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) lds[wave][lane][v] = acc[v].store();
+        __syncthreads();
+        if (wave == 0 && active) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                if (v < nvalid) {
+                    A a;
+                    a.load(lds[0][lane][v]);
+                    for (int w = 1; w < 4; ++w) {
+                        A b;
+                        b.load(lds[w][lane][v]);
+                        a.merge(b);
+                    }
+                    const uint32_t o = ko * p.lf + j0 + v;
+                    if (p.splits == 1) finish(p, a, o);
+                    else reinterpret_cast<Store*>(p.partials)[size_t(o) * p.splits + sp] = a.store();
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- split combine
+template <int AK, int OP>
+__global__ __launch_bounds__(256) void reduce_combine_kernel(const RedPlan p) {
+    using A = Acc<AK, OP>;
+    for (uint32_t o = blockIdx.x * 256 + threadIdx.x; o < p.n_out; o += gridDim.x * 256) {
+        const Store* part = reinterpret_cast<const Store*>(p.partials) + size_t(o) * p.splits;
+        A a;
+        a.load(part[0]);
+        for (int s = 1; s < p.splits; ++s) {
+            A b;
+            b.load(part[s]);
+            a.merge(b);
+        }
+        finish(p, a, o);
+    }
+}
+
+}  // namespace xg
+
+namespace xg {
+
+using KFn = void (*)(const RedPlan);
+
+struct RedKernels {
+    KFn rows32, rows64, cols32, cols64;
+};
+
+// specialised programs: unrolled x4, 32-bit indexing only (64-bit plans run the generic kernels)
+template <int V, int NR, class S, int AK, int OP> RedKernels spec_kernels_for() {
+    return RedKernels{reduce_rows_kernel<V, NR, S, false, AK, OP, 4>, nullptr,
+                      reduce_cols_kernel<V, NR, S, false, AK, OP, 4>, nullptr};
+}
+
+// generic interpreter: V = 4, no unrolling
+template <int AK, int OP> RedKernels dyn_kernels_for() {
+    return RedKernels{reduce_rows_kernel<4, kMaxRegs, DynSpec, false, AK, OP, 1>,
+                      reduce_rows_kernel<4, kMaxRegs, DynSpec, true, AK, OP, 1>,
+                      reduce_cols_kernel<4, kMaxRegs, DynSpec, false, AK, OP, 1>,
+                      reduce_cols_kernel<4, kMaxRegs, DynSpec, true, AK, OP, 1>};
+}
+
+// generic kernels per accumulator family (defined in reduce_dyn_*.hip)
+bool dyn_reduce_float(int ak, int op, RedKernels* k);
+bool dyn_reduce_int32(int ak, int op, RedKernels* k);
+bool dyn_reduce_int64(int ak, int op, RedKernels* k);
+KFn combine_kernel(int ak, int op);
+
+}  // namespace xg

@@ -1,0 +1,16 @@
+// scan_float.hip — scan kernel instantiations for one accumulator family.
+#include "scan_kernels.hpp"
+
+namespace xg {
+
+bool scan_kernels_float(int ak, int op, ScanKernels* k) {
+#define XG_S(AK, OP) if (ak == AK && op == OP) { *k = scan_kernels<AK, OP>(); return true; }
+    XG_S(AK_F32C, RO_ADD)
+    XG_S(AK_F64C, RO_ADD)
+    XG_S(AK_F32, RO_MUL)
+    XG_S(AK_F64, RO_MUL)
+#undef XG_S
+    return false;
+}
+
+}  // namespace xg

@@ -1,0 +1,160 @@
+// scan_kernels.hpp — scan kernel templates (see scan.hip for the design).
+//
+// Replaces accumulator_impl's copy-then-sequential recurrence
+// (reducers/xaccumulator.hpp:215-297).  Rows layout (scan axis contiguous
+// in the output): a workgroup owns a row and walks it in tiles of
+// 256 x V elements; each thread scans its V elements, a wave scan (shuffles)
+// and an LDS pass over the four wave totals give every thread its exclusive
+// prefix, and the tile total carries into the next tile.  Cols layout (scan
+// axis strided): a thread owns V adjacent columns and walks the axis.
+// Float sums use the compensated accumulator of the reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "acc.hpp"
+#include "chunk.hpp"
+#include "scan.hpp"
+
+namespace xg {
+
+template <int V, bool I64>
+__device__ __forceinline__ void scan_store(const ScanPlan& p, const uint32_t* idx, uint32_t j0, int nvalid,
+                                           const Reg (&val)[V]) {
+    const int odt = p.out_dtype;
+    const int esz = dtype_size(odt);
+    auto off = unit_offset<I64>(p.out_stride, idx, p.rank, j0);
+    char* ptr = p.out_ptr + int64_t(off) * esz;
+    Reg o[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) o[v] = odt == XG_BFLOAT16 ? cvt(val[v], p.rdt, XG_FLOAT32) : cvt(val[v], p.rdt, odt);
+    if (p.pad_ == LM_CONTIG && nvalid == V) {
+        store_vec<V>(ptr, esz, odt, o);
+    } else {
+        const int64_t os = p.out_stride[p.rank - 1];
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+            if (v < nvalid) store_raw(ptr + int64_t(v) * os * esz, esz, to_storage(o[v], odt));
+    }
+}
+
+template <int V, int NR, class S, bool I64, int AK, int OP>
+__global__ __launch_bounds__(256) void scan_rows_kernel(const ScanPlan p) {
+    using A = Acc<AK, OP>;
+    __shared__ Store wtot[4];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    for (uint32_t row = blockIdx.x; row < p.rows; row += gridDim.x) {
+        uint32_t idx[kMaxRank];
+#pragma unroll
+        for (int k = 0; k < kMaxRank; ++k) idx[k] = 0;
+        uint32_t q = row;
+        for (int k = p.rank - 2; k >= 0; --k) {
+            uint32_t d = fastdiv(q, p.div[k]);
+            idx[k] = q - d * uint32_t(p.shape[k]);
+            q = d;
+        }
+        A carry;
+        carry.init(p.identity);
+        for (uint32_t base = 0; base < p.len; base += 256u * V) {
+            const uint32_t j0 = base + threadIdx.x * V;
+            const int nvalid = j0 < p.len ? min(V, int(p.len - j0)) : 0;
+            Reg vals[V];
+            if (nvalid > 0) eval_chunk<V, NR, S, I64>(p.leaves, p.prog, p.rank, idx, j0, nvalid, vals);
+            // thread-local inclusive scan
+            A run[V];
+            A t;
+            t.init(p.identity);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                if (v < nvalid) t.add(vals[v]);
+                run[v] = t;
+            }
+            // wave inclusive scan of thread totals (earlier values merge first)
+            A x = t;
+            for (int d = 1; d < 64; d <<= 1) {
+                A y = x.shfl_up(d);
+                if (lane >= d) {
+                    y.merge(x);
+                    x = y;
+                }
+            }
+            if (lane == 63) wtot[wave] = x.store();
+            A ex = x.shfl_up(1);
+            if (lane == 0) ex.init(p.identity);
+            __syncthreads();
+            A pre = carry;
+            for (int w = 0; w < wave; ++w) {
+                A o;
+                o.load(wtot[w]);
+                pre.merge(o);
+            }
+            pre.merge(ex);
+            Reg outv[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                A o = pre;
+                o.merge(run[v]);
+                outv[v] = o.value(p.rdt);
+            }
+            if (nvalid > 0) scan_store<V, I64>(p, idx, j0, nvalid, outv);
+            for (int w = 0; w < 4; ++w) {
+                A o;
+                o.load(wtot[w]);
+                carry.merge(o);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int V, int NR, class S, bool I64, int AK, int OP>
+__global__ __launch_bounds__(256) void scan_cols_kernel(const ScanPlan p) {
+    using A = Acc<AK, OP>;
+    for (uint32_t item = blockIdx.x * 256 + threadIdx.x; item < p.rows; item += gridDim.x * 256) {
+        uint32_t oo = fastdiv(item, p.div_nchunks_in);
+        const uint32_t j0 = (item - oo * p.nchunks_in) * V;
+        const int nvalid = min(V, int(p.inner - j0));
+        uint32_t idx[kMaxRank];
+#pragma unroll
+        for (int k = 0; k < kMaxRank; ++k) idx[k] = 0;
+        for (int k = p.rank - 3; k >= 0; --k) {
+            uint32_t d = fastdiv(oo, p.div[k]);
+            idx[k] = oo - d * uint32_t(p.shape[k]);
+            oo = d;
+        }
+        A acc[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v].init(p.identity);
+        for (uint32_t i = 0; i < p.len; ++i) {
+            idx[p.rank - 2] = i;
+            Reg vals[V];
+            eval_chunk<V, NR, S, I64>(p.leaves, p.prog, p.rank, idx, j0, nvalid, vals);
+            Reg outv[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                acc[v].add(vals[v]);
+                outv[v] = acc[v].value(p.rdt);
+            }
+            scan_store<V, I64>(p, idx, j0, nvalid, outv);
+        }
+    }
+}
+
+using SFn = void (*)(const ScanPlan);
+
+struct ScanKernels {
+    SFn rows32, rows64, cols32, cols64;
+};
+
+template <int AK, int OP> ScanKernels scan_kernels() {
+    return ScanKernels{scan_rows_kernel<4, kMaxRegs, DynSpec, false, AK, OP>,
+                       scan_rows_kernel<4, kMaxRegs, DynSpec, true, AK, OP>,
+                       scan_cols_kernel<4, kMaxRegs, DynSpec, false, AK, OP>,
+                       scan_cols_kernel<4, kMaxRegs, DynSpec, true, AK, OP>};
+}
+
+bool scan_kernels_float(int ak, int op, ScanKernels* k);
+bool scan_kernels_int(int ak, int op, ScanKernels* k);
+
+}  // namespace xg

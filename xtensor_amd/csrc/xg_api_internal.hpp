@@ -1,0 +1,39 @@
+// xg_api_internal.hpp — helpers shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <exception>
+#include <new>
+#include <string>
+
+#include "planner.hpp"
+
+namespace xg {
+
+void set_error(const std::string& m);
+int device_cus();
+void note_kernel(const char* name);
+void check_hip(hipError_t e, const char* what);
+// per (device, stream) scratch buffer of at least `bytes`
+void* workspace(size_t bytes, hipStream_t stream);
+
+// Run f, turning thrown errors into a status code and the thread's message.
+template <class F> xg_status guard(F&& f) {
+    try {
+        f();
+        set_error("");
+        return XG_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return xg_status(e.code);
+    } catch (const std::bad_alloc&) {
+        set_error("host out of memory");
+        return XG_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return XG_INVALID_ARGUMENT;
+    }
+}
+
+}  // namespace xg
