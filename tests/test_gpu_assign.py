@@ -54,23 +54,29 @@ def make_ab(rows=2048, cols=1024, seed=0):
 
 @pytest.mark.parametrize("rows,cols", [(2048, 1024), (33, 17), (1, 1)])
 def test_mul_add_exp_broadcast_f64(rows, cols):
-    """c = a*b + exp(-a*a), b broadcast over rows (f64)."""
+    """c = a*b + exp(-a*a), b broadcast over rows (f64).
+
+    exp is the only inexact operator: the device's exp(-a*a) is within 2 ulp
+    of the host libm's, and c is bit-exact given that exp (products and sum
+    are IEEE operations, each rounded once).  Where a*b nearly cancels
+    exp(-a*a) an ulp of exp is many ulps of c, so c itself is compared through
+    its exp term."""
     a_h, b_h = make_ab(rows, cols)
     A_, B_ = Dual.of(a_h), Dual.of(b_h)
     c = xt.empty((rows, cols), "float64")
-    e = A_.dev * B_.dev + xt.exp(-A_.dev * A_.dev)
-    xt.noalias(c).assign(e)
+    xt.noalias(c).assign(A_.dev * B_.dev + xt.exp(-A_.dev * A_.dev))
     sync()
     used = xt.last_kernel()
-    ref = oracle_for(A_, B_).assign(A_.dev * B_.dev + xt.exp(-A_.dev * A_.dev))
     got = c.to_numpy()
-    u = ulp_diff(got, ref)
-    assert u.max() <= 2, f"max ulp {u.max()}"
-    frac = np.count_nonzero(u) / u.size
-    print(f"kernel={used} mismatching elements {frac:.2e}, max ulp {u.max()}")
-    # the products and the sum are exact: recompute with the oracle's exp replaced by ours
-    ex = xt.eval(xt.exp(-A_.dev * A_.dev)).to_numpy()
-    assert same_bits(a_h * b_h + ex, got)
+    o = oracle_for(A_, B_)
+    ex_dev = xt.eval(xt.exp(-A_.dev * A_.dev)).to_numpy()
+    ex_ref = o.assign(xt.exp(-A_.dev * A_.dev))
+    u = ulp_diff(ex_dev, ex_ref)
+    assert u.max() <= 2, f"exp: max ulp {u.max()}"
+    assert same_bits(a_h * b_h + ex_dev, got)
+    ref = o.assign(A_.dev * B_.dev + xt.exp(-A_.dev * A_.dev))
+    frac = np.count_nonzero(got != ref) / got.size
+    print(f"kernel={used}: exp max ulp {u.max()}, c differs from the host in {frac:.2e} of elements")
 
 
 def test_spec_and_dynamic_agree_bitwise():
@@ -308,6 +314,20 @@ def test_transcendentals_within_2ulp(fn, dt):
     sync()
     ref = oracle_for(X).assign(e)
     u = ulp_diff(got, ref)
+    if fn == "cbrt" and dt == "float64":
+        # the host libm's double cbrt is itself up to 3 ulp from the exact cube
+        # root on these inputs; the device value is checked against the exact
+        # root (60-digit decimal) instead: within 1 ulp
+        import decimal
+        decimal.getcontext().prec = 60
+        third = decimal.Decimal(1) / decimal.Decimal(3)
+        worst = 0.0
+        for xv, gv in zip(v[:1024], got[:1024]):
+            t = decimal.Decimal(float(xv)) ** third
+            worst = max(worst, abs(float((decimal.Decimal(float(gv)) - t) / decimal.Decimal(float(np.spacing(gv))))))
+        assert worst <= 1.0, f"cbrt/float64: {worst} ulp from the exact root"
+        assert u.max() <= 4
+        return
     assert u.max() <= 2, f"{fn}/{dt}: max {u.max()} ulp"
 
 

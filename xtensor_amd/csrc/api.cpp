@@ -124,10 +124,22 @@ xg_status xg_expr_info(const xg_expr* e, int32_t* dtype, int32_t* ndim, int64_t*
     });
 }
 
+// plan with the compiled-kernel vector width; without a matching compiled
+// program, re-plan for the generic interpreter (V = 4)
+static EwPlan plan_assign_best(const xg_expr* e, const xg_tensor* out, int* V) {
+    *V = choose_vector_width(e, out);
+    EwPlan p = plan_assign(e, out, *V);
+    if (*V != 4 && !ew_find_spec(p, *V)) {
+        *V = 4;
+        p = plan_assign(e, out, 4);
+    }
+    return p;
+}
+
 xg_status xg_assign(const xg_expr* e, const xg_tensor* out, xg_stream_t stream) {
     return guard([&] {
-        int V = choose_vector_width(e, out);
-        EwPlan p = plan_assign(e, out, V);
+        int V = 4;
+        EwPlan p = plan_assign_best(e, out, &V);
         if (p.units == 0) {
             note_kernel("empty");
             return;
@@ -216,8 +228,8 @@ xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out,
         tmp.data = workspace(size_t(n) * dtype_size(dt), st);
         check_hip(hipSuccess, "");
         {
-            int Vw = choose_vector_width(e, &tmp);
-            EwPlan ep = plan_assign(e, &tmp, Vw);
+            int Vw = 4;
+            EwPlan ep = plan_assign_best(e, &tmp, &Vw);
             check_hip(launch_ew(ep, Vw, device_cus(), st, nullptr), "scan copy launch");
         }
         // 1-d view of the temporary as the only leaf
@@ -259,9 +271,8 @@ xg_status xg_plan_assign(const xg_expr* e, const xg_tensor* out, xg_plan_t* plan
         if (!plan) raise(XG_INVALID_ARGUMENT, "null plan pointer");
         auto* p = new xg_plan_s();
         p->kind = 0;
-        p->V = choose_vector_width(e, out);
         try {
-            p->ew = plan_assign(e, out, p->V);
+            p->ew = plan_assign_best(e, out, &p->V);
         } catch (...) {
             delete p;
             throw;
@@ -338,8 +349,8 @@ void xg_debug_force_dynamic(int on) { ew_set_force_dynamic(on); }
 // Name of the compiled specialisation an assignment would use, "" if none.
 xg_status xg_debug_assign_kernel(const xg_expr* e, const xg_tensor* out, char* name, int32_t cap) {
     return guard([&] {
-        int V = choose_vector_width(e, out);
-        EwPlan p = plan_assign(e, out, V);
+        int V = 4;
+        EwPlan p = plan_assign_best(e, out, &V);
         const char* s = ew_find_spec(p, V);
         std::string n = s ? s : "";
         if (cap > 0) {

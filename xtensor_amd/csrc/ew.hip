@@ -62,6 +62,92 @@ __global__ __launch_bounds__(256) void ew_kernel(const EwPlan plan) {
     }
 }
 
+// ---------------------------------------------------------------- compiled programs
+// Kernels instantiated on a constexpr program, for normalised rank RANK <= 4
+// and 32-bit offsets.  Every loop is unrolled at compile time, leaf types
+// and access modes are constants, so each leaf read is one address
+// computation and one (vector) load.
+
+template <int RANK, bool WITH_J>
+__device__ __forceinline__ int32_t coff(const int32_t (&st)[4], const uint32_t (&idx)[4], uint32_t j0) {
+    int32_t off = WITH_J ? int32_t(j0) * st[RANK - 1] : 0;
+#pragma unroll
+    for (int k = 0; k < RANK - 1; ++k) off += int32_t(idx[k]) * st[k];
+    return off;
+}
+
+template <int V>
+__device__ __forceinline__ void load_at(const char* p, int mode, int dt, int32_t is, int nvalid, Reg (&out)[V]) {
+    const int esz = dtype_size(dt);
+    if (mode == LM_BCAST) {
+        Reg x = from_storage(load_raw(p, esz), dt);
+#pragma unroll
+        for (int v = 0; v < V; ++v) out[v] = x;
+    } else if (mode == LM_CONTIG && nvalid == V) {
+        load_vec<V>(p, esz, dt, out);
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+            out[v] = v < nvalid ? from_storage(load_raw(p + int64_t(v) * is * esz, esz), dt) : 0;
+    }
+}
+
+template <class S, int I, int V, int NR, int RANK>
+__device__ __forceinline__ void run_spec(const EwPlan& plan, const uint32_t (&idx)[4], uint32_t j0, int nvalid,
+                                         Reg (&R)[NR][V]) {
+    if constexpr (I < S::nops) {
+        constexpr MicroOp m = S::ops[I];
+        if constexpr (m.op == MO_LOAD) {
+            constexpr int dt = S::leaf_dtype[m.aux];
+            constexpr int mode = S::leaf_mode[m.aux];
+            const int32_t off = coff<RANK, mode != LM_BCAST>(plan.cst[m.aux], idx, j0);
+            const char* p = plan.leaves[m.aux].ptr + int64_t(off) * dtype_size(dt);
+            load_at<V>(p, mode, dt, plan.cst[m.aux][RANK - 1], nvalid, R[m.dst]);
+        } else {
+            Reg tmp[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) tmp[v] = exec_scalar(m, R[m.a][v], R[m.b][v], R[m.c][v], plan.prog.consts);
+#pragma unroll
+            for (int v = 0; v < V; ++v) R[m.dst][v] = tmp[v];
+        }
+        run_spec<S, I + 1, V, NR, RANK>(plan, idx, j0, nvalid, R);
+    }
+}
+
+template <int V, int NR, class S, int RANK>
+__global__ __launch_bounds__(256) void ew_spec_kernel(const EwPlan plan) {
+    const uint32_t nunits = uint32_t(plan.units);
+    const uint32_t step = gridDim.x * 256u;
+    for (uint32_t u = blockIdx.x * 256u + threadIdx.x; u < nunits; u += step) {
+        const uint32_t row = fastdiv(u, plan.div_chunks);
+        const uint32_t j0 = (u - row * plan.nchunks) * V;
+        uint32_t idx[4] = {0, 0, 0, 0};
+        uint32_t r = row;
+#pragma unroll
+        for (int k = RANK - 2; k >= 1; --k) {
+            const uint32_t q = fastdiv(r, plan.div[k]);
+            idx[k] = r - q * plan.shape32[k];
+            r = q;
+        }
+        if constexpr (RANK >= 2) idx[0] = r;
+        const int nvalid = min(V, int(plan.inner - j0));
+        Reg R[NR][V];
+        run_spec<S, 0, V, NR, RANK>(plan, idx, j0, nvalid, R);
+        constexpr int odt = S::out_dtype;
+        constexpr int esz = dtype_size(odt);
+        const int32_t off = coff<RANK, true>(plan.cst[kMaxLeaves], idx, j0);
+        char* p = plan.out_ptr + int64_t(off) * esz;
+        if (S::out_mode == LM_CONTIG && nvalid == V) {
+            store_vec<V>(p, esz, odt, R[S::result_reg]);
+        } else {
+            const int32_t os = plan.cst[kMaxLeaves][RANK - 1];
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+                if (v < nvalid) store_raw(p + int64_t(v) * os * esz, esz, to_storage(R[S::result_reg][v], odt));
+        }
+    }
+}
+
 }  // namespace xg
 
 // compiled specialisations (generated from specs_list.cpp at build time)
@@ -71,12 +157,13 @@ namespace xg {
 
 namespace {
 
+using EwFn = void (*)(const EwPlan);
+
 struct SpecEntry {
     const char* name;
     int V;
     SpecKey key;
-    void (*k32)(const EwPlan);
-    void (*k64)(const EwPlan);
+    EwFn by_rank[4];  // compiled kernels for normalised rank 1..4
 };
 
 template <class S> SpecKey key_of() {
@@ -98,8 +185,10 @@ std::vector<SpecEntry>& spec_table() {
     static std::vector<SpecEntry> t;
     static std::once_flag once;
     std::call_once(once, [] {
-#define XG_REGISTER_SPEC(S) \
-    t.push_back(SpecEntry{#S, S::V, key_of<S>(), ew_kernel<S::V, S::nregs, S, false>, ew_kernel<S::V, S::nregs, S, true>});
+#define XG_REGISTER_SPEC(S)                                                                                   \
+    t.push_back(SpecEntry{#S, S::V, key_of<S>(),                                                              \
+                          {ew_spec_kernel<S::V, S::nregs, S, 1>, ew_spec_kernel<S::V, S::nregs, S, 2>,          \
+                           ew_spec_kernel<S::V, S::nregs, S, 3>, ew_spec_kernel<S::V, S::nregs, S, 4>}});
         XG_SPEC_LIST(XG_REGISTER_SPEC)
 #undef XG_REGISTER_SPEC
     });
@@ -113,17 +202,18 @@ int g_force_dynamic = 0;
 void ew_set_force_dynamic(int on) { g_force_dynamic = on; }
 
 const char* ew_find_spec(const EwPlan& p, int V) {
-    if (g_force_dynamic) return nullptr;
+    if (g_force_dynamic || !p.compact) return nullptr;
     SpecKey k = spec_key(p);
     for (auto& s : spec_table())
         if (s.V == V && spec_equal(s.key, k)) return s.name;
     return nullptr;
 }
 
+// one work unit per thread (the loop in the kernels only runs beyond 2^31 blocks)
 int ew_grid(uint64_t units, int cus) {
+    (void)cus;
     uint64_t blocks = (units + 255) / 256;
-    uint64_t cap = uint64_t(cus) * 16;
-    if (blocks > cap) blocks = cap;
+    if (blocks > 0x7fffffffull) blocks = 0x7fffffffull;
     if (blocks < 1) blocks = 1;
     return int(blocks);
 }
@@ -131,24 +221,20 @@ int ew_grid(uint64_t units, int cus) {
 hipError_t launch_ew(const EwPlan& p, int V, int cus, hipStream_t stream, const char** used) {
     if (p.units == 0) return hipSuccess;
     dim3 grid(ew_grid(p.units, cus)), block(256);
-    if (!g_force_dynamic) {
+    if (!g_force_dynamic && p.compact) {
         SpecKey k = spec_key(p);
         for (auto& s : spec_table()) {
             if (s.V == V && spec_equal(s.key, k)) {
                 if (used) *used = s.name;
-                hipLaunchKernelGGL(p.index64 ? s.k64 : s.k32, grid, block, 0, stream, p);
+                hipLaunchKernelGGL(s.by_rank[p.rank - 1], grid, block, 0, stream, p);
                 return hipGetLastError();
             }
         }
     }
+    if (V != 4) return hipErrorInvalidValue;  // the generic interpreter is built for V = 4
     if (used) *used = "dynamic";
-    if (V == 8) {
-        if (p.index64) hipLaunchKernelGGL((ew_kernel<8, kMaxRegs, DynSpec, true>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((ew_kernel<8, kMaxRegs, DynSpec, false>), grid, block, 0, stream, p);
-    } else {
-        if (p.index64) hipLaunchKernelGGL((ew_kernel<4, kMaxRegs, DynSpec, true>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((ew_kernel<4, kMaxRegs, DynSpec, false>), grid, block, 0, stream, p);
-    }
+    if (p.index64) hipLaunchKernelGGL((ew_kernel<4, kMaxRegs, DynSpec, true>), grid, block, 0, stream, p);
+    else hipLaunchKernelGGL((ew_kernel<4, kMaxRegs, DynSpec, false>), grid, block, 0, stream, p);
     return hipGetLastError();
 }
 

@@ -159,6 +159,18 @@ template <class T> __device__ __forceinline__ Reg flt_binop(int op, Reg x, Reg y
     }
 }
 
+// cube root in double: the device library's value refined by one Newton step
+// whose residual y^3 - x is evaluated with an exact product (y*y split by
+// FMA), which brings it within 1 ulp of the exact cube root
+__device__ __forceinline__ double cbrt_refined(double x) {
+    double y = cbrt(x);
+    if (!(y != 0.0) || !isfinite(y)) return y;
+    double y2 = y * y;
+    double y2e = fma(y, y, -y2);          // y*y = y2 + y2e exactly
+    double r = fma(y2, y, -x) + y2e * y;  // y^3 - x
+    return y - r / (3.0 * y2);
+}
+
 template <class T> __device__ __forceinline__ Reg flt_unop(int op, Reg x) {
     T a = fget<T>(x);
     switch (op) {
@@ -172,7 +184,9 @@ template <class T> __device__ __forceinline__ Reg flt_unop(int op, Reg x) {
     case MO_LOG10: return fput<T>(log10(a));
     case MO_LOG1P: return fput<T>(log1p(a));
     case MO_SQRT: return fput<T>(sqrt(a));
-    case MO_CBRT: return fput<T>(cbrt(a));
+    case MO_CBRT:
+        if constexpr (sizeof(T) == 8) return fput<T>(cbrt_refined(a));
+        else return fput<T>(cbrt(a));
     case MO_SIN: return fput<T>(sin(a));
     case MO_COS: return fput<T>(cos(a));
     case MO_TAN: return fput<T>(tan(a));

@@ -518,9 +518,11 @@ Program lower(const xg_expr* e, const ExprInfo& info, int final_dt, int map) {
 }
 
 int choose_vector_width(const xg_expr* e, const xg_tensor* out) {
-    bool wide = out && dtype_size(out->dtype) == 8;
-    for (int l = 0; e && l < e->n_leaves; ++l) wide = wide || dtype_size(e->leaves[l].dtype) == 8;
-    return wide ? 4 : 8;
+    // each lane moves 16 bytes of the widest operand per access, so a wave's
+    // access is 1 KiB of consecutive memory
+    int w = out ? dtype_size(out->dtype) : 1;
+    for (int l = 0; e && l < e->n_leaves; ++l) w = std::max(w, dtype_size(e->leaves[l].dtype));
+    return std::max(2, std::min(16, 16 / w));
 }
 
 // ------------------------------------------------------------------ assign plan
@@ -640,6 +642,14 @@ EwPlan plan_assign(const xg_expr* e, const xg_tensor* out, int V) {
         lv.mode = is == 0 ? LM_BCAST : aligned(lv.ptr, esz, is, lv.stride) ? LM_CONTIG : LM_STRIDED;
     }
     p.index64 = max_off >= (int64_t(1) << 31) ? 1 : 0;
+    p.compact = (!p.index64 && rank <= 4) ? 1 : 0;
+    if (p.compact) {
+        for (int k = 0; k < rank; ++k) {
+            p.shape32[k] = uint32_t(p.shape[k]);
+            p.cst[kMaxLeaves][k] = int32_t(p.out_stride[k]);
+            for (int pl = 0; pl < p.prog.nleaves; ++pl) p.cst[pl][k] = int32_t(p.leaves[pl].stride[k]);
+        }
+    }
     return p;
 }
 

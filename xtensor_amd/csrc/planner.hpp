@@ -47,7 +47,8 @@ std::vector<std::vector<int64_t>> leaf_strides(const xg_expr* e, const Shape& it
 
 bool broadcastable(const Shape& src, const Shape& dst);
 
-// Elements per work unit: 8, or 4 when an operand has 8-byte elements.
+// Elements per work unit of compiled kernels: 16 bytes of the widest operand
+// (the generic interpreter always uses 4).
 int choose_vector_width(const xg_expr* e, const xg_tensor* out);
 
 // Elementwise plan for out = e, vector width V.

@@ -368,7 +368,8 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
         int64_t sa = dom >= 0 ? a.leaf[dom] : 0, sb = dom >= 0 ? b.leaf[dom] : 0;
         return sa > sb;
     };
-    bool fast_red = fast < 0 || dims[size_t(fast)].red || kept.empty();
+    // an empty reduction reads nothing: rows layout with zero chunks per output
+    bool fast_red = empty_red || fast < 0 || dims[size_t(fast)].red || kept.empty();
     std::vector<RDim> order;
     RDim unit{};
     unit.n = 1;
@@ -469,6 +470,9 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
         p.splits = int(std::max<uint64_t>(1, std::min<uint64_t>({want, maxs, traffic, 65535})));
         p.per_split = uint32_t((rows + p.splits - 1) / p.splits);
     }
+    while (p.splits > 1 && uint64_t(n_out) * uint64_t(p.splits) >= (uint64_t(1) << 31)) p.splits /= 2;
+    if (p.layout == RL_ROWS) p.per_split = uint32_t((uint64_t(p.red_count) + p.splits - 1) / p.splits);
+    else p.per_split = uint32_t((uint64_t(p.red_count) + p.splits - 1) / p.splits);
     p.div_splits = make_fastdiv(uint32_t(p.splits));
     return p;
 }

@@ -159,6 +159,11 @@ struct EwPlan {
     int64_t out_stride[kMaxRank];
     LeafView leaves[kMaxLeaves];
     Program prog;
+    // compact form for compiled kernels (rank <= 4, 32-bit offsets):
+    // cst[l] = strides of leaf l, cst[kMaxLeaves] = output strides
+    int32_t compact;
+    uint32_t shape32[4];
+    int32_t cst[kMaxLeaves + 1][4];
 };
 
 }  // namespace xg
