@@ -297,6 +297,67 @@ size_t xg_plan_workspace_size(xg_plan_t plan);
 xg_status xg_plan_execute(xg_plan_t plan, void* workspace, xg_stream_t stream);
 void xg_plan_destroy(xg_plan_t plan);
 
+/* ---- batches of uneven instances -------------------------------------------
+ * Many independent 2-D instances of one row-wise template, evaluated by ONE
+ * persistent launch (a work queue over row blocks of all instances), which
+ * can be replayed as a HIP graph.  Per instance this replaces the plain
+ * expression sequence
+ *   XG_BATCH_SOFTMAX: m = amax(x, {1}, keep_dims); e = exp(x - m);
+ *                     out = e / sum(e, {1}, keep_dims)
+ *                     (the row softmax of test/test_xmath.cpp:985-1027,
+ *                      three assign/reduce calls of core/xmath.hpp:1803-1823)
+ *   XG_BATCH_SUM:     out = sum(x, {1})        (core/xmath.hpp:1803)
+ *   XG_BATCH_AMAX:    out = amax(x, {1})       (core/xmath.hpp:1823)
+ * Floating-point sums accumulate with TwoSum compensation in the input's
+ * precision; a row's result depends only on that row (its reduction tree is
+ * a function of `cols` alone), so it is the same whatever batch it is in.
+ * dtypes: float32 and float64 for all ops, int32 for SUM (wrapping, as
+ * sum(int32) in xtensor) and AMAX. */
+typedef enum xg_batch_op {
+    XG_BATCH_SOFTMAX = 0,
+    XG_BATCH_SUM = 1,
+    XG_BATCH_AMAX = 2
+} xg_batch_op;
+
+typedef struct xg_batch_item {
+    const void* in;          /* rows x cols, row stride in_row_stride          */
+    void* out;               /* SOFTMAX: rows x cols; SUM/AMAX: rows values    */
+    int64_t rows;
+    int64_t cols;
+    int64_t in_row_stride;   /* elements between consecutive input rows        */
+    int64_t out_row_stride;  /* SOFTMAX: elements between output rows;
+                                SUM/AMAX: elements between output values      */
+} xg_batch_item;
+
+/* Launch shape; chosen per shape class by xg_batch_autotune. */
+typedef struct xg_batch_tuning {
+    int32_t block;           /* threads per workgroup (64..256, multiple of 64) */
+    int32_t blocks_per_cu;   /* persistent workgroups per compute unit          */
+    int32_t task_elems;      /* target elements per work-queue task             */
+    int32_t flags;
+} xg_batch_tuning;
+
+typedef struct xg_batch_s* xg_batch_t;
+
+/* Validate the items and upload the work tables (the library allocates
+ * them; the caller keeps owning in/out).  Tuning starts from the cached
+ * choice for the batch's shape class, or a default. */
+xg_status xg_batch_create(int32_t op, int32_t dtype, const xg_batch_item* items, int64_t n_items,
+                          xg_batch_t* batch);
+/* Time the candidate launch shapes on `stream` (each trial writes the
+ * outputs), keep the fastest and cache it for the shape class. */
+xg_status xg_batch_autotune(xg_batch_t batch, xg_stream_t stream);
+xg_status xg_batch_get_tuning(xg_batch_t batch, xg_batch_tuning* tuning);
+xg_status xg_batch_set_tuning(xg_batch_t batch, const xg_batch_tuning* tuning);
+/* Enqueue the batch as kernel launches (may be called under graph capture). */
+xg_status xg_batch_launch(xg_batch_t batch, xg_stream_t stream);
+/* Enqueue the batch by replaying its instantiated HIP graph.  One batch
+ * must not run on two streams at once (its work queue is shared). */
+xg_status xg_batch_execute(xg_batch_t batch, xg_stream_t stream);
+/* Number of work-queue tasks and total elements of the batch. */
+xg_status xg_batch_info(xg_batch_t batch, int64_t* n_tasks, int64_t* n_elements);
+void xg_batch_destroy(xg_batch_t batch);
+
 #ifdef __cplusplus
 }
 #endif

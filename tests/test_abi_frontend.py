@@ -51,6 +51,7 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(xg_tensor), sizeof(xg_scalar), sizeof(xg_node),
          sizeof(xg_expr), sizeof(xg_reduce_desc), sizeof(xg_scan_desc));
   printf("%zu %zu %zu\n", offsetof(xg_reduce_desc, initial), offsetof(xg_expr, nodes), offsetof(xg_tensor, shape));
+  printf("%zu %zu %zu\n", sizeof(xg_batch_item), sizeof(xg_batch_tuning), offsetof(xg_batch_item, out_row_stride));
   return 0;
 }
 '''
@@ -63,8 +64,21 @@ int main(void) {
     sizes = [int(x) for x in out]
     mine = [C.sizeof(A.Tensor), C.sizeof(A.Scalar), C.sizeof(A.Node), C.sizeof(A.Expr),
             C.sizeof(A.ReduceDesc), C.sizeof(A.ScanDesc),
-            A.ReduceDesc.initial.offset, A.Expr.nodes.offset, A.Tensor.shape.offset]
+            A.ReduceDesc.initial.offset, A.Expr.nodes.offset, A.Tensor.shape.offset,
+            C.sizeof(A.BatchItem), C.sizeof(A.BatchTuning), A.BatchItem.out_row_stride.offset]
     assert sizes == mine
+
+
+def test_batch_argument_errors_without_device():
+    """Op and dtype are validated before any device call."""
+    lib = A.lib()
+    h = C.c_void_p()
+    items = (A.BatchItem * 1)()
+    assert lib.xg_batch_create(7, A.FLOAT32, items, 1, C.byref(h)) == A.INVALID_ARGUMENT
+    assert lib.xg_batch_create(0, A.INT32, items, 1, C.byref(h)) == A.TYPE_ERROR
+    assert b"dtype" in lib.xg_last_error_message()
+    assert lib.xg_batch_create(0, A.FLOAT32, None, 3, C.byref(h)) == A.INVALID_ARGUMENT
+    assert lib.xg_batch_execute(None, None) == A.INVALID_ARGUMENT
 
 
 # ---------------------------------------------------------------- type fuzz

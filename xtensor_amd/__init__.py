@@ -18,4 +18,6 @@ from .core import (  # noqa: F401
     tanh, transpose, trunc, view, where, zeros,
 )
 
+from .batch import Batch  # noqa: F401,E402
+
 __version__ = "0.1.0"

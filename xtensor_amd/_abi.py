@@ -103,6 +103,16 @@ class ScanDesc(C.Structure):
     _fields_ = [("op", C.c_int32), ("acc_dtype", C.c_int32), ("axis", C.c_int32), ("flags", C.c_int32)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("in_", C.c_void_p), ("out", C.c_void_p), ("rows", C.c_int64), ("cols", C.c_int64),
+                ("in_row_stride", C.c_int64), ("out_row_stride", C.c_int64)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("task_elems", C.c_int32),
+                ("flags", C.c_int32)]
+
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libxg.so")
 
@@ -124,6 +134,14 @@ EXPORTS = {
     "xg_plan_workspace_size": (C.c_size_t, [C.c_void_p]),
     "xg_plan_execute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "xg_plan_destroy": (None, [C.c_void_p]),
+    "xg_batch_create": (C.c_int, [C.c_int32, C.c_int32, _P(BatchItem), C.c_int64, _P(C.c_void_p)]),
+    "xg_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "xg_batch_get_tuning": (C.c_int, [C.c_void_p, _P(BatchTuning)]),
+    "xg_batch_set_tuning": (C.c_int, [C.c_void_p, _P(BatchTuning)]),
+    "xg_batch_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "xg_batch_execute": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "xg_batch_info": (C.c_int, [C.c_void_p, _P(C.c_int64), _P(C.c_int64)]),
+    "xg_batch_destroy": (None, [C.c_void_p]),
 }
 
 # diagnostic entry points (not part of include/xg.h)

@@ -1,0 +1,71 @@
+// batch.hpp — batched evaluation of many uneven instances in one persistent
+// launch (row-wise templates: softmax, sum, max over the last axis).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "xg_internal.hpp"
+
+namespace xg {
+
+enum BatchOp : int32_t { BO_SOFTMAX = XG_BATCH_SOFTMAX, BO_SUM = XG_BATCH_SUM, BO_MAX = XG_BATCH_AMAX };
+
+// one instance as seen by the kernel
+struct BItem {
+    const char* in;
+    char* out;
+    int64_t in_rs;   // row stride of the input, elements
+    int64_t out_rs;  // row stride of the output, elements (sum/max: output element stride)
+    int32_t rows;
+    int32_t cols;
+    int32_t lpr;     // lanes per row (4..64), a function of cols only
+    int32_t pad_;
+};
+
+// one unit of work: `nrows` consecutive rows of one instance, processed by one wave
+struct BTask {
+    int32_t item;
+    int32_t row0;
+    int32_t nrows;
+    int32_t pad_;
+};
+
+struct BatchArgs {
+    const BItem* items;
+    const BTask* tasks;
+    uint32_t ntasks;
+    uint32_t* counter;  // work-queue head, reset to 0 before each launch
+};
+
+// lanes per row for a row of `cols` elements; this fixes the per-row
+// reduction tree, so it must depend on nothing but `cols`
+inline int lanes_per_row(int64_t cols) {
+    if (cols <= 4) return 4;
+    if (cols <= 8) return 8;
+    if (cols <= 16) return 16;
+    if (cols <= 32) return 32;
+    return 64;
+}
+
+// Elements per lane a row of `cols` is held in (a power of two), or 0 when
+// the row is longer than the register budget (32 four-byte or 16 eight-byte
+// values per lane) and is streamed from memory instead.
+XG_HD constexpr int batch_ne_class(int cols, int lpr, int esz) {
+    const int need = (cols + lpr - 1) / lpr;
+    const int cap = esz == 8 ? 16 : 32;
+    if (need > cap) return 0;
+    int ne = 1;
+    while (ne < need) ne <<= 1;
+    return ne;
+}
+
+// Row groups (of 64 / lpr rows each) a wave holds at once for elements-per-lane
+// class `ne`: enough to keep about `cap` values per lane in flight, at most 8.
+XG_HD constexpr int batch_rows_at_once(int ne, int esz) {
+    const int cap = esz == 8 ? 8 : 32;
+    if (ne == 0) return 1;
+    const int rb = cap / ne;
+    return rb < 1 ? 1 : (rb > 8 ? 8 : rb);
+}
+
+}  // namespace xg
