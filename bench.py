@@ -85,18 +85,15 @@ def cpu_baseline(np, budget_s=10.0):
                       "restatement of xtensor's path (the reference itself is not buildable here)"}
 
 
-def uneven_batch(np, torch, xt, steps=20, n_inst=4096):
-    """4096 float32 instances, rows ~ log-uniform [1,512], cols ~ log-uniform
-    [1,2048] (seed 3), x ~ N(0,2); row softmax of every instance.  Batched:
-    one autotuned persistent launch replayed as a HIP graph.  Plain: three
-    prepared plans per instance (amax keep_dims, fused exp-sum, normalise)."""
+def make_uneven_batch(np, torch, xt, n_inst=4096):
+    """The uneven batch: instances packed at 256-byte boundaries in one HBM
+    buffer per side; returns the prepared Batch and its buffers."""
     from xtensor_amd.batch import Batch
     from xtensor_amd.workloads import uneven_shapes, uneven_values
 
     shapes = uneven_shapes(n_inst, seed=3)
     xs = uneven_values(shapes, seed=3)
     elems = sum(r * c for r, c in shapes)
-    # one HBM buffer per side, instances packed at 256-byte boundaries
     offs, o = [], 0
     for r, c in shapes:
         offs.append(o)
@@ -109,7 +106,15 @@ def uneven_batch(np, torch, xt, steps=20, n_inst=4096):
     xin.storage.copy_(torch.from_numpy(host.view(np.uint8)))
     ins = [xt.DeviceArray(xin.storage, "float32", s, None, off) for off, s in zip(offs, shapes)]
     outs = [xt.DeviceArray(xout.storage, "float32", s, None, off) for off, s in zip(offs, shapes)]
-    b = Batch("softmax", ins, outs)
+    return Batch("softmax", ins, outs), ins, outs, offs, xs, xout, elems
+
+
+def uneven_batch(np, torch, xt, steps=20, n_inst=4096):
+    """4096 float32 instances, rows ~ log-uniform [1,512], cols ~ log-uniform
+    [1,2048] (seed 3), x ~ N(0,2); row softmax of every instance.  Batched:
+    one autotuned persistent launch replayed as a HIP graph.  Plain: three
+    prepared plans per instance (amax keep_dims, fused exp-sum, normalise)."""
+    b, ins, outs, offs, xs, xout, elems = make_uneven_batch(np, torch, xt, n_inst)
     tuning = b.autotune()
     b.execute()
     torch.cuda.synchronize()
@@ -168,6 +173,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile-batch", action="store_true",
+                    help="profiling pass for rocprofv3: the uneven softmax batch, one launch and 3 graph replays")
     ap.add_argument("--profile", action="store_true",
                     help="profiling pass for rocprofv3: a 256 MiB float32 copy (counter calibration) "
                          "and the workload kernel, 3 launches each, nothing else")
@@ -183,6 +190,16 @@ def main():
     from xtensor_amd import _abi as A
 
     torch.cuda.set_device(0)
+    if args.profile_batch:
+        b, *_rest, elems = make_uneven_batch(np, torch, xt)   # default tuning: no trial launches
+        b.launch()
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.execute()
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_pass": "uneven_batch_softmax_f32", "elements": elems,
+                          "algorithmic_bytes_each": 8 * elems, "tuning": b.tuning, "tasks": b.info()["tasks"]}))
+        return
     x_h, mean_h, var_h, gamma_h, beta_h = make_inputs(np)
     params = [xt.from_numpy(v) for v in (mean_h, var_h, gamma_h, beta_h)]
     x0 = xt.from_numpy(x_h)

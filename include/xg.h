@@ -333,8 +333,10 @@ typedef struct xg_batch_item {
 typedef struct xg_batch_tuning {
     int32_t block;           /* threads per workgroup (64..256, multiple of 64) */
     int32_t blocks_per_cu;   /* persistent workgroups per compute unit          */
-    int32_t task_elems;      /* target elements per work-queue task             */
-    int32_t flags;
+    int32_t task_elems;      /* target elements per task                        */
+    int32_t dyn_percent;     /* share of the work (smallest tasks, percent)
+                                handed out by the shared work queue; the rest
+                                is assigned to waves on the host              */
 } xg_batch_tuning;
 
 typedef struct xg_batch_s* xg_batch_t;

@@ -175,8 +175,9 @@ def test_same_alone_and_in_the_mix():
     b.launch()
     sync()
     mixed = [o.to_numpy() for o in outs]
-    for t in ({"block": 64, "blocks_per_cu": 1, "task_elems": 64},
-              {"block": 256, "blocks_per_cu": 16, "task_elems": 32768}):
+    for t in ({"block": 64, "blocks_per_cu": 1, "task_elems": 64, "dyn_percent": 0},
+              {"block": 256, "blocks_per_cu": 16, "task_elems": 32768, "dyn_percent": 100},
+              {"block": 128, "blocks_per_cu": 3, "task_elems": 1000, "dyn_percent": 37}):
         outs2 = [xt.zeros(x.shape, "float32") for x in xs]
         b2 = Batch("softmax", dx, outs2)
         b2.tuning = t
@@ -219,7 +220,7 @@ def test_autotune_keeps_results():
     sync()
     before = [o.to_numpy() for o in outs]
     t = b.autotune()
-    assert t["block"] in (64, 128, 256)
+    assert t["block"] in (128, 256) and 0 <= t["dyn_percent"] <= 100
     b.execute()
     sync()
     for a, o in zip(before, outs):

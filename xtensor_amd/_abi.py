@@ -110,7 +110,7 @@ class BatchItem(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("task_elems", C.c_int32),
-                ("flags", C.c_int32)]
+                ("dyn_percent", C.c_int32)]
 
 
 HERE = os.path.dirname(os.path.abspath(__file__))

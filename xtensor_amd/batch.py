@@ -76,13 +76,14 @@ class Batch:
     def tuning(self) -> dict:
         t = A.BatchTuning()
         _check(A.lib().xg_batch_get_tuning(self.handle, C.byref(t)))
-        return {"block": t.block, "blocks_per_cu": t.blocks_per_cu, "task_elems": t.task_elems}
+        return {"block": t.block, "blocks_per_cu": t.blocks_per_cu, "task_elems": t.task_elems,
+                "dyn_percent": t.dyn_percent}
 
     @tuning.setter
     def tuning(self, d: dict) -> None:
         cur = self.tuning
         cur.update(d)
-        t = A.BatchTuning(cur["block"], cur["blocks_per_cu"], cur["task_elems"], 0)
+        t = A.BatchTuning(cur["block"], cur["blocks_per_cu"], cur["task_elems"], cur["dyn_percent"])
         _check(A.lib().xg_batch_set_tuning(self.handle, C.byref(t)))
 
     def info(self) -> dict:

@@ -3,17 +3,19 @@
 // The plain path evaluates a row softmax of one instance as three launches
 // (amax with keep_dims, the fused exp-sum reduction, the normalising
 // assignment; the pattern of test/test_xmath.cpp:985-1027).  Here a single
-// kernel handles a whole batch: waves take tasks (a few rows of one
-// instance) from a work queue (an atomic counter), load each row once into
-// registers and write the result once.  A row of `cols` elements is owned
-// by `lanes_per_row(cols)` lanes; its maximum is exact, its sum is a
-// compensated (TwoSum) sum folded in a fixed butterfly order, so the value
-// of every row depends only on that row — never on the batch around it.
+// persistent kernel handles a whole batch: every wave works through a list
+// of tasks (a few rows of one instance) that the host balanced across the
+// grid, then takes left-over tasks from a shared work queue (an atomic
+// counter); it loads each row once into registers and writes the result
+// once.  A row of `cols` elements is owned by `lanes_per_row(cols)` lanes;
+// its maximum is exact, its sum is a compensated (TwoSum) sum folded in a
+// fixed butterfly order, so the value of every row depends only on that
+// row — never on the batch around it or on the schedule.
 //
-// Host side: task tables sorted largest first (the queue hands the long
-// tasks out early, the short ones fill the tail), a two-node HIP graph
-// (queue reset, kernel) per batch, and an autotuner over the launch shape
-// whose choice is cached per shape class.
+// Host side: task split and schedule (longest task first onto the least
+// loaded wave), a two-node HIP graph (queue reset, kernel) per batch, and an
+// autotuner over task size, launch shape and queue share whose choice is
+// cached per shape class.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,6 +26,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <queue>
 #include <tuple>
 #include <vector>
 
@@ -63,6 +66,15 @@ template <> struct BTraits<int32_t> {
 // starting from numeric_limits<T>::lowest() (core/xmath.hpp:573-602, 1823)
 template <class T> __device__ __forceinline__ T fmax_x(T acc, T x) { return acc > x ? acc : x; }
 
+// One-instruction maximum (IEEE maxNum: a NaN operand yields the other).
+// Used for the softmax shift only: a NaN anywhere in a row makes that row's
+// sum, and so every output of the row, NaN either way.
+template <class T> __device__ __forceinline__ T max_num(T a, T b) {
+    if constexpr (std::is_same_v<T, float>) return __builtin_fmaxf(a, b);
+    else if constexpr (std::is_same_v<T, double>) return __builtin_fmax(a, b);
+    else return fmax_x(a, b);
+}
+
 // butterfly combine inside aligned groups of `lpr` lanes, lower lane first
 template <class A> __device__ __forceinline__ A sub_combine(A a, const int lpr) {
     const int lane = threadIdx.x & 63;
@@ -87,18 +99,34 @@ template <class T> __device__ __forceinline__ T sub_max(T m, int lpr) {
     return m;
 }
 
-// exp(x - m) with the rounding error of the subtraction put back:
-// d + e == x - m exactly (TwoSum), exp(x - m) = exp(d) * (1 + e + O(e^2))
+// exp(x - m) for x <= m (m is the row maximum), without the rounding error
+// of forming x - m in the input precision: d + e == x - m exactly (TwoSum).
+//   float:  2^p with p = (d + e) * log2(e) carried as ph + pl (Dekker
+//           product of d and log2(e) split in two floats, plus the small
+//           terms), p = n + f with n = rint(ph), 2^f by the hardware exp2 on
+//           |f| <= 0.5 + tiny, scaled by 2^n.  Float arithmetic only.
+//   double: exp(d) * (1 + e).
+// x - m is in [-inf, 0] or NaN; -inf (and anything below the float range)
+// gives 0, NaN stays NaN.
 template <class T> __device__ __forceinline__ T exp_shifted(T x, T m) {
-    using Tr = BTraits<T>;
     const T nm = -m;
     const T d = x + nm;
     const T bp = d - x;
-    T e = (x - (d - bp)) + (nm - bp);
-    const T v = Tr::ex(d);
-    if (!(d == d) || d - d != T(0)) e = T(0);  // d infinite or NaN: no correction
-    if constexpr (std::is_same_v<T, float>) return __builtin_fmaf(v, e, v);
-    else return __builtin_fma(v, e, v);
+    const T e = (x - (d - bp)) + (nm - bp);
+    if constexpr (std::is_same_v<T, float>) {
+        constexpr float L = 1.44269502f;         // log2(e) rounded to float
+        constexpr float Llo = 1.92596303e-08f;    // log2(e) - L
+        const float ph = d * L;
+        const float pl = __builtin_fmaf(d, L, -ph) + __builtin_fmaf(d, Llo, e * L);
+        const float n = __builtin_rintf(ph);
+        const float f = (ph - n) + pl;
+        // the clamp keeps the exponent conversion defined for d = -inf / NaN
+        const float r = __builtin_ldexpf(__builtin_amdgcn_exp2f(f), int(__builtin_fmaxf(n, -200.f)));
+        return d < -160.f ? 0.f : r;
+    } else {
+        const double v = exp(d);
+        return (d >= -DBL_MAX) ? __builtin_fma(v, e, v) : v;
+    }
 }
 
 // Divide by a row's sum: q = RN(v * r) with r = 1 / sum, then one
@@ -156,9 +184,13 @@ __device__ __forceinline__ void rows_regs(const BItem& it, int nrows, int row0, 
         const uint32_t loff = (uint32_t(sub) * uint32_t(it.in_rs) + uint32_t(sl)) * E;
 #pragma unroll
         for (int k = 0; k < NE; ++k) {
-            // element sl + k * LPR; the lane-dependent part stays on the left
-            // of the comparison so that no per-k lane value is kept live
-            const bool in = sl < cols - k * LPR;
+            // element sl + k * LPR.  For NE >= 2 the class guarantees
+            // cols > (NE / 2) * LPR, so the first half is always inside the
+            // row.  (Skipping chunks wholly past the row end with a uniform
+            // branch was measured slower: it splits the batch of loads.)
+            // The lane-dependent part stays on the left of the comparison so
+            // that no per-k lane value is kept live.
+            const bool in = (NE >= 2 && k < NE / 2) || sl < cols - k * LPR;
             const T x = bload<T>(rin[g], loff + uint32_t(k * LPR) * E);
             if constexpr (OP == BO_SUM) v[g][k] = (PER == 1 || in) ? x : T(0);
             else v[g][k] = in ? x : Tr::lowest();
@@ -170,7 +202,7 @@ __device__ __forceinline__ void rows_regs(const BItem& it, int nrows, int row0, 
         for (int g = 0; g < RB; ++g) {
             m[g] = v[g][0];
 #pragma unroll
-            for (int k = 1; k < NE; ++k) m[g] = fmax_x(m[g], v[g][k]);
+            for (int k = 1; k < NE; ++k) m[g] = OP == BO_SOFTMAX ? max_num(m[g], v[g][k]) : fmax_x(m[g], v[g][k]);
         }
 #pragma unroll
         for (int g = 0; g < RB; ++g) m[g] = sub_max(m[g], LPR);
@@ -188,7 +220,10 @@ __device__ __forceinline__ void rows_regs(const BItem& it, int nrows, int row0, 
             s[g].init(0);
 #pragma unroll
             for (int k = 0; k < NE; ++k) {
-                if constexpr (OP == BO_SOFTMAX) v[g][k] = sl < cols - k * LPR ? exp_shifted(v[g][k], m[g]) : T(0);
+                if constexpr (OP == BO_SOFTMAX) {
+                    const bool in = (NE >= 2 && k < NE / 2) || sl < cols - k * LPR;
+                    v[g][k] = in ? exp_shifted(v[g][k], m[g]) : T(0);
+                }
                 s[g].add(Tr::reg(v[g][k]));
             }
         }
@@ -273,11 +308,17 @@ __device__ __forceinline__ void rows_empty(const BItem& it, int nrows, int row0,
 template <class T, int OP>
 __global__ __launch_bounds__(256, 5) void batch_rows_kernel(const BatchArgs a) {
     const int lane = threadIdx.x & 63;
-    for (;;) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(a.counter, 1u);
-        t = __builtin_amdgcn_readlane(t, 0);  // wave-uniform from here on
-        if (t >= a.ntasks) break;             // every wave leaves once the queue is drained
+    const uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    uint32_t t = a.wbeg[w];
+    uint32_t tend = a.wbeg[w + 1];
+    for (;; ++t) {
+        if (t >= tend) {  // own list done: take from the shared queue
+            uint32_t q = 0;
+            if (lane == 0) q = atomicAdd(a.counter, 1u);
+            t = a.dyn_begin + __builtin_amdgcn_readlane(q, 0);  // wave-uniform from here on
+            if (t >= a.ntasks) break;  // every wave leaves once the queue is drained
+            tend = t + 1;
+        }
         const BTask task = a.tasks[t];
         const BItem it = a.items[task.item];
         const int nrows = task.nrows;
@@ -336,7 +377,7 @@ std::map<ClassKey, xg_batch_tuning> g_tune_cache;
 
 int ilog2(double x) { return x < 1.0 ? 0 : int(std::floor(std::log2(x))); }
 
-xg_batch_tuning default_tuning() { return xg_batch_tuning{256, 8, 8192, 0}; }
+xg_batch_tuning default_tuning() { return xg_batch_tuning{256, 4, 2048, 10}; }
 
 void check_tuning(const xg_batch_tuning& t) {
     if (t.block < 64 || t.block > 256 || t.block % 64 != 0)
@@ -345,6 +386,8 @@ void check_tuning(const xg_batch_tuning& t) {
         raise(XG_INVALID_ARGUMENT, "batch tuning: blocks_per_cu must be in [1, 64]");
     if (t.task_elems < 64 || t.task_elems > (1 << 24))
         raise(XG_INVALID_ARGUMENT, "batch tuning: task_elems must be in [64, 2^24]");
+    if (t.dyn_percent < 0 || t.dyn_percent > 100)
+        raise(XG_INVALID_ARGUMENT, "batch tuning: dyn_percent must be in [0, 100]");
 }
 
 }  // namespace
@@ -366,6 +409,10 @@ struct xg_batch_s {
     BItem* d_items = nullptr;
     BTask* d_tasks = nullptr;
     size_t d_tasks_cap = 0;
+    std::vector<uint32_t> wbeg;
+    uint32_t* d_wbeg = nullptr;
+    size_t d_wbeg_cap = 0;
+    uint32_t dyn_begin = 0;
     uint32_t* d_counter = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -376,9 +423,11 @@ struct xg_batch_s {
         drop_graph();
         if (d_items) (void)hipFree(d_items);
         if (d_tasks) (void)hipFree(d_tasks);
+        if (d_wbeg) (void)hipFree(d_wbeg);
         if (d_counter) (void)hipFree(d_counter);
         d_items = nullptr;
         d_tasks = nullptr;
+        d_wbeg = nullptr;
         d_counter = nullptr;
     }
 
@@ -389,43 +438,103 @@ struct xg_batch_s {
         graph = nullptr;
     }
 
-    BatchArgs args() const { return BatchArgs{d_items, d_tasks, uint32_t(tasks.size()), d_counter}; }
+    BatchArgs args() const {
+        return BatchArgs{d_items, d_tasks, d_wbeg, dyn_begin, uint32_t(tasks.size()), d_counter};
+    }
+    int grid_blocks() const { return std::max(1, cus * tune.blocks_per_cu); }
+    int grid_waves() const { return grid_blocks() * (tune.block / 64); }
 
-    // split every instance into tasks of about tune.task_elems elements
-    // (whole row groups of one wave), largest first
-    void build_tasks() {
+    // estimated work of a task: register slots it occupies (padded rows x
+    // padded row length) plus a fixed cost per row-group iteration
+    int64_t task_cost(const BTask& t) const {
+        const BItem& it = items[size_t(t.item)];
+        const int esz = dtype_size(dtype);
+        const int ne = batch_ne_class(it.cols, it.lpr, esz);
+        const int64_t per = 64 / it.lpr;
+        const int64_t step = per * batch_rows_at_once(ne, esz);
+        const int64_t groups = (t.nrows + per - 1) / per;
+        const int64_t slots = ne ? int64_t(ne) * it.lpr : int64_t(it.cols) * 3;  // streamed rows: 3 passes
+        return groups * std::max<int64_t>(slots, 64) + 512 * ((t.nrows + step - 1) / step);
+    }
+
+    // Split every instance into tasks of about tune.task_elems elements
+    // (whole row-group iterations of one wave), then schedule them: all but
+    // the smallest tune.dyn_percent of the work is assigned longest first to
+    // the least loaded wave of the grid (no atomics, balanced to within one
+    // task); the rest, largest first, goes to the shared work queue.
+    void build_schedule() {
         tasks.clear();
+        const int esz = dtype_size(dtype);
         for (size_t i = 0; i < items.size(); ++i) {
             const BItem& it = items[i];
             if (it.rows == 0) continue;
             if (it.cols == 0 && op == BO_SOFTMAX) continue;
-            const int esz = dtype_size(dtype);
-            const int64_t per = int64_t(64 / it.lpr) * batch_rows_at_once(batch_ne_class(it.cols, it.lpr, esz), esz);
-            int64_t want = std::max<int64_t>(1, tune.task_elems / std::max<int64_t>(1, it.cols));
-            int64_t rpt = ((want + per - 1) / per) * per;
+            const int64_t step = int64_t(64 / it.lpr) * batch_rows_at_once(batch_ne_class(it.cols, it.lpr, esz), esz);
+            const int64_t want = std::max<int64_t>(1, tune.task_elems / std::max<int64_t>(1, it.cols));
+            const int64_t rpt = ((want + step - 1) / step) * step;
             for (int64_t r0 = 0; r0 < it.rows; r0 += rpt) {
                 const int64_t n = std::min<int64_t>(rpt, it.rows - r0);
                 tasks.push_back(BTask{int32_t(i), int32_t(r0), int32_t(n), 0});
             }
         }
         if (tasks.size() >= (size_t(1) << 31)) raise(XG_UNSUPPORTED, "batch has too many tasks");
-        auto cost = [&](const BTask& t) {
-            const BItem& it = items[size_t(t.item)];
-            const int64_t per = 64 / it.lpr;
-            return ((t.nrows + per - 1) / per) * std::max<int64_t>(it.lpr, int64_t(it.cols));
-        };
-        std::stable_sort(tasks.begin(), tasks.end(), [&](const BTask& a, const BTask& b) { return cost(a) > cost(b); });
-        if (tasks.size() > d_tasks_cap) {
-            drop_graph();
-            if (d_tasks) check_hip(hipFree(d_tasks), "hipFree");
-            d_tasks = nullptr;
-            d_tasks_cap = 0;
-            if (hipMalloc(&d_tasks, std::max<size_t>(1, tasks.size()) * sizeof(BTask)) != hipSuccess)
-                raise(XG_OUT_OF_MEMORY, "batch task table allocation failed");
-            d_tasks_cap = std::max<size_t>(1, tasks.size());
+        std::vector<int64_t> cost(tasks.size());
+        std::vector<size_t> order(tasks.size());
+        int64_t total = 0;
+        for (size_t k = 0; k < tasks.size(); ++k) {
+            cost[k] = task_cost(tasks[k]);
+            order[k] = k;
+            total += cost[k];
         }
-        if (!tasks.empty())
-            check_hip(hipMemcpy(d_tasks, tasks.data(), tasks.size() * sizeof(BTask), hipMemcpyHostToDevice), "hipMemcpy");
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+        // the dynamic share: a suffix of the sorted list
+        size_t nstatic = order.size();
+        {
+            const int64_t dyn_target = total * int64_t(tune.dyn_percent) / 100;
+            int64_t acc = 0;
+            while (nstatic > 0 && acc + cost[order[nstatic - 1]] <= dyn_target) acc += cost[order[--nstatic]];
+        }
+        // longest processing time first onto the least loaded wave
+        const int W = grid_waves();
+        std::vector<std::vector<uint32_t>> lists(static_cast<size_t>(W));
+        {
+            using Load = std::pair<int64_t, int>;
+            std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+            for (int w = 0; w < W; ++w) heap.push({0, w});
+            for (size_t k = 0; k < nstatic; ++k) {
+                Load l = heap.top();
+                heap.pop();
+                lists[size_t(l.second)].push_back(uint32_t(order[k]));
+                l.first += cost[order[k]];
+                heap.push(l);
+            }
+        }
+        std::vector<BTask> sched;
+        sched.reserve(tasks.size());
+        wbeg.assign(size_t(W) + 1, 0);
+        for (int w = 0; w < W; ++w) {
+            wbeg[size_t(w)] = uint32_t(sched.size());
+            for (uint32_t k : lists[size_t(w)]) sched.push_back(tasks[k]);
+        }
+        wbeg[size_t(W)] = uint32_t(sched.size());
+        dyn_begin = uint32_t(sched.size());
+        for (size_t k = nstatic; k < order.size(); ++k) sched.push_back(tasks[order[k]]);
+        tasks.swap(sched);
+        upload(d_tasks, d_tasks_cap, tasks);
+        upload(d_wbeg, d_wbeg_cap, wbeg);
+    }
+
+    template <class V> void upload(V*& dptr, size_t& cap, const std::vector<V>& h) {
+        if (h.size() > cap) {
+            drop_graph();
+            if (dptr) check_hip(hipFree(dptr), "hipFree");
+            dptr = nullptr;
+            cap = 0;
+            if (hipMalloc(&dptr, std::max<size_t>(1, h.size()) * sizeof(V)) != hipSuccess)
+                raise(XG_OUT_OF_MEMORY, "batch schedule allocation failed");
+            cap = std::max<size_t>(1, h.size());
+        }
+        if (!h.empty()) check_hip(hipMemcpy(dptr, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice), "hipMemcpy");
     }
 
     void enqueue(hipStream_t stream) const {
@@ -433,8 +542,7 @@ struct xg_batch_s {
         hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, stream, d_counter);
         check_hip(hipGetLastError(), "batch reset launch");
         if (tasks.empty()) return;
-        const int blocks = std::max(1, cus * tune.blocks_per_cu);
-        hipLaunchKernelGGL(k, dim3(blocks), dim3(tune.block), 0, stream, args());
+        hipLaunchKernelGGL(k, dim3(grid_blocks()), dim3(tune.block), 0, stream, args());
         check_hip(hipGetLastError(), "batch kernel launch");
     }
 
@@ -458,7 +566,7 @@ struct xg_batch_s {
             void* kargs[] = {&a};
             hipKernelNodeParams kp{};
             kp.func = reinterpret_cast<void*>(batch_kernel_for(op, dtype));
-            kp.gridDim = dim3(unsigned(std::max(1, cus * tune.blocks_per_cu)));
+            kp.gridDim = dim3(unsigned(grid_blocks()));
             kp.blockDim = dim3(unsigned(tune.block));
             kp.sharedMemBytes = 0;
             kp.kernelParams = kargs;
@@ -471,10 +579,9 @@ struct xg_batch_s {
 
     void apply(const xg_batch_tuning& t) {
         check_tuning(t);
-        const bool retask = t.task_elems != tune.task_elems || tasks.empty();
         tune = t;
         drop_graph();
-        if (retask) build_tasks();
+        build_schedule();
     }
 };
 
@@ -558,7 +665,7 @@ xg_status xg_batch_create(int32_t op, int32_t dtype, const xg_batch_item* items,
             check_hip(hipMemcpy(b->d_items, b->items.data(), b->items.size() * sizeof(BItem), hipMemcpyHostToDevice), "hipMemcpy");
         check_tuning(t);
         b->tune = t;
-        b->build_tasks();
+        b->build_schedule();
         *out = b.release();
     });
 }
@@ -567,9 +674,12 @@ xg_status xg_batch_autotune(xg_batch_t b, xg_stream_t stream) {
     return guard([&] {
         if (!b) raise(XG_INVALID_ARGUMENT, "null batch");
         hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-        static const int kBlocks[] = {64, 128, 256};
-        static const int kPerCu[] = {2, 4, 8, 16};
-        static const int kElems[] = {2048, 8192, 32768};
+        // Waves per CU up to what fits at once: the host-side schedule
+        // assumes every wave of the grid is resident from the start.
+        static const int kBlocks[] = {128, 256};
+        static const int kWavesPerCu[] = {8, 12, 16, 20, 24, 32};
+        static const int kElems[] = {1024, 2048, 4096, 8192};
+        static const int kDyn[] = {0, 10, 30};
         hipEvent_t e0, e1;
         check_hip(hipEventCreate(&e0), "hipEventCreate");
         check_hip(hipEventCreate(&e1), "hipEventCreate");
@@ -578,8 +688,15 @@ xg_status xg_batch_autotune(xg_batch_t b, xg_stream_t stream) {
         try {
             for (int te : kElems) {
                 for (int blk : kBlocks) {
-                    for (int pc : kPerCu) {
-                        xg_batch_tuning t{blk, pc, te, 0};
+                    int resident = 0;
+                    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                  &resident, reinterpret_cast<const void*>(batch_kernel_for(b->op, b->dtype)), blk, 0),
+                              "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+                    for (int wpc : kWavesPerCu) {
+                      const int pc = wpc / (blk / 64);
+                      if (pc < 1 || pc > std::max(1, resident)) continue;
+                      for (int dp : kDyn) {
+                        xg_batch_tuning t{blk, pc, te, dp};
                         b->apply(t);
                         b->enqueue(s);  // warm-up
                         check_hip(hipEventRecord(e0, s), "hipEventRecord");
@@ -592,6 +709,7 @@ xg_status xg_batch_autotune(xg_batch_t b, xg_stream_t stream) {
                             best_ms = ms;
                             best = t;
                         }
+                      }
                     }
                 }
             }

@@ -30,11 +30,17 @@ struct BTask {
     int32_t pad_;
 };
 
+// Schedule: tasks[wbeg[w], wbeg[w + 1]) belong to wave w of the grid
+// (assigned on the host, longest first onto the least loaded wave); the
+// tasks from dyn_begin on are handed out by a work queue to waves that have
+// finished their own list.
 struct BatchArgs {
     const BItem* items;
     const BTask* tasks;
+    const uint32_t* wbeg;  // grid waves + 1 entries
+    uint32_t dyn_begin;
     uint32_t ntasks;
-    uint32_t* counter;  // work-queue head, reset to 0 before each launch
+    uint32_t* counter;     // work-queue head, reset to 0 before each launch
 };
 
 // lanes per row for a row of `cols` elements; this fixes the per-row
