@@ -167,12 +167,111 @@ def uneven_batch(np, torch, xt, steps=20, n_inst=4096):
     }
 
 
+SERVE_TEMPLATES = {  # template -> dtypes it is defined for
+    "affine_clip": ("float32", "float64"),
+    "row_sum": ("float32", "float64", "int32"),
+    "row_softmax": ("float32", "float64"),
+    "bitmix": ("int32",),
+}
+
+
+def serve_requests(np, n_req, seed=4):
+    """Request mix: template uniform over the four, total elements log-uniform
+    in [1e3, 1e8], dtype uniform over the template's dtypes; shapes
+    affine_clip [N, 16, 8, 8], rows x 1024 (sum) / 2048 (softmax), bitmix [n]."""
+    rng = np.random.default_rng(seed)
+    names = list(SERVE_TEMPLATES)
+    out = []
+    for _ in range(n_req):
+        tpl = names[rng.integers(len(names))]
+        dts = SERVE_TEMPLATES[tpl]
+        dt = dts[rng.integers(len(dts))]
+        n = int(np.exp(rng.uniform(np.log(1e3), np.log(1e8))))
+        if tpl == "affine_clip":
+            shape, oshape = (max(1, n // 1024), 16, 8, 8), None
+        elif tpl == "bitmix":
+            shape, oshape = (n,), None
+        else:
+            cols = min(n, 1024 if tpl == "row_sum" else 2048)
+            shape = (max(1, n // cols), cols)
+            oshape = (shape[0],) if tpl == "row_sum" else None
+        out.append((tpl, dt, shape, oshape or shape))
+    return out
+
+
+def serving(np, torch, xt, rates=(1000, 10000, 50000), n_req=300, seed=4):
+    """Served request stream: Poisson arrivals at each rate (seed 4), operands
+    resident in HBM, 64 GB admission budget; continuous batching (4 streams,
+    up to 64 requests per launch) vs the plain per-request path (one stream,
+    each request alone through the per-expression calls)."""
+    from xtensor_amd.serve import Server, now_ns
+
+    reqs = serve_requests(np, n_req, seed)
+    esz = {"float32": 4, "float64": 8, "int32": 4}
+    # one resident input pool per (template, dtype), requests read a prefix of it
+    maxn = {}
+    for tpl, dt, shape, _ in reqs:
+        maxn[(tpl, dt)] = max(maxn.get((tpl, dt), 0), int(np.prod(shape)))
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    pools, params = {}, {}
+    for (tpl, dt), n in maxn.items():
+        if dt == "int32":
+            t = torch.randint(-2**20, 2**20, (n,), dtype=torch.int32, device="cuda", generator=gen)
+        else:
+            t = torch.randn(n, dtype=getattr(torch, dt), device="cuda", generator=gen) * 2
+        pools[(tpl, dt)] = t
+        if tpl == "affine_clip":
+            pd = getattr(torch, dt)
+            params[dt] = [torch.rand(16, dtype=pd, device="cuda", generator=gen) * s + o
+                          for s, o in ((0.2, -0.1), (1.5, 0.5), (1.0, 0.5), (0.2, -0.1))]
+    nbytes = []
+    for tpl, dt, shape, oshape in reqs:
+        nbytes.append((int(np.prod(shape)) + int(np.prod(oshape))) * esz[dt])
+
+    def view(t, shape):
+        return xt.DeviceArray(t.view(torch.uint8), str(t.dtype).replace("torch.", ""), shape)
+
+    result = {"requests_per_rate": n_req, "budget_GB": 64, "mean_request_MB": round(float(np.mean(nbytes)) / 1e6, 2)}
+    for rate in rates:
+        gaps = np.random.default_rng(seed + rate).exponential(1.0 / rate, n_req)
+        for mode in ("batched", "plain"):
+            outs = [torch.empty(int(np.prod(o)), dtype=getattr(torch, dt), device="cuda") for _, dt, _, o in reqs]
+            cfg = dict(budget_bytes=64 << 30, queue_capacity=1 << 20)
+            cfg.update(dict(streams=4, max_batch=64) if mode == "batched" else dict(streams=1, max_batch=1, plain=True))
+            torch.cuda.synchronize()
+            with Server(**cfg) as srv:
+                t0 = now_ns() + 2_000_000
+                arr = t0 + np.cumsum(gaps * 1e9).astype(np.int64)
+                for k, (tpl, dt, shape, oshape) in enumerate(reqs):
+                    while now_ns() < arr[k]:
+                        pass
+                    x = view(pools[(tpl, dt)][:int(np.prod(shape))], shape)
+                    o = view(outs[k], oshape)
+                    ps = [view(p, (16,)) for p in params[dt]] if tpl == "affine_clip" else []
+                    srv.submit(tpl, x, o, ps, tag=k, arrival_ns=int(arr[k]))
+                srv.drain(600_000)
+                comps = srv.poll(n_req + 16)
+                st = srv.stats()
+            lat = np.array([c["done_ns"] - c["arrival_ns"] for c in comps]) / 1e6
+            span = (max(c["done_ns"] for c in comps) - int(arr[0])) / 1e9
+            result[f"{mode}_{rate}"] = {
+                "req_per_s": round(n_req / span, 1), "GBps": round(sum(nbytes) / span / 1e9, 1),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "launches": st["launches"], "peak_inflight_GB": round(st["peak_inflight_bytes"] / 1e9, 2),
+                "ok": int(sum(c["status"] == 0 for c in comps)),
+            }
+            print(f"served stream {mode} at {rate} req/s: {result[f'{mode}_{rate}']}", file=sys.stderr, flush=True)
+            del outs
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--profile-batch", action="store_true",
                     help="profiling pass for rocprofv3: the uneven softmax batch, one launch and 3 graph replays")
     ap.add_argument("--profile", action="store_true",
@@ -252,6 +351,8 @@ def main():
     extra["int32_bitmix_GBps"] = round(N * C * H * W * 8 / (ikms * 1e-3) / 1e9, 1)
     del xi, yi, ip
     extra["uneven_batch_softmax_f32"] = uneven_batch(np, torch, xt)
+    if not args.no_serve:
+        extra["served_stream"] = serving(np, torch, xt)
 
     line = {
         "metric": "fused expr+axis-reduce GB/s (% of 8 TB/s); batched inst/s + p99 latency",

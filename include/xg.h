@@ -295,6 +295,13 @@ size_t xg_plan_workspace_size(xg_plan_t plan);
 /* Enqueue the plan on `stream`; `workspace` must hold xg_plan_workspace_size
  * bytes (NULL: the library's per-stream workspace is used). */
 xg_status xg_plan_execute(xg_plan_t plan, void* workspace, xg_stream_t stream);
+/* Enqueue n plans that do not read each other's outputs.  Elementwise plans
+ * that run the same compiled program at the same normalised rank are fused
+ * into one launch (their descriptors go to the device in one asynchronous
+ * upload through a pinned buffer of the stream); the rest run one launch
+ * each, with the library's per-stream workspace.  No counterpart in xtensor:
+ * this is the batching path for many small expressions. */
+xg_status xg_plan_execute_many(const xg_plan_t* plans, int64_t n, xg_stream_t stream);
 void xg_plan_destroy(xg_plan_t plan);
 
 /* ---- batches of uneven instances -------------------------------------------
@@ -359,6 +366,100 @@ xg_status xg_batch_execute(xg_batch_t batch, xg_stream_t stream);
 /* Number of work-queue tasks and total elements of the batch. */
 xg_status xg_batch_info(xg_batch_t batch, int64_t* n_tasks, int64_t* n_elements);
 void xg_batch_destroy(xg_batch_t batch);
+
+/* ---- request server ---------------------------------------------------------
+ * A request loop for serving: callers submit requests whose operands already
+ * live in device memory; a library thread admits them in arrival order while
+ * the device bytes held by admitted, unfinished requests stay within a
+ * budget, groups admitted requests by template, dtype and shape class, and
+ * runs each group as one fused launch on one of a few streams, so groups
+ * overlap on the device and new requests are admitted as earlier ones
+ * finish (continuous batching).  No counterpart in xtensor, which evaluates
+ * synchronously on the calling thread; each template is the expression the
+ * comment names, with xtensor's semantics:
+ *   XG_TPL_AFFINE_CLIP: out = clip((x - mean) / sqrt(var + 1e-5) * gamma + beta, -3, 3)
+ *                       x, out [N, C, H, W]; mean/var/gamma/beta [C] broadcast
+ *                       as [1, C, 1, 1]; float32 or float64
+ *   XG_TPL_ROW_SUM:     out = sum(x, {1}), x [rows, cols]; float32/float64/int32
+ *   XG_TPL_ROW_SOFTMAX: out = softmax of each row of x [rows, cols]; float32/float64
+ *   XG_TPL_BITMIX:      out = (x * 3 + (x >> 2)) & 0xFFFF, x [n]; int32
+ * All operands are contiguous row-major.  Results do not depend on what else
+ * is in flight: the elementwise templates are exact functions of their
+ * element, and a row's reduction order depends only on its length. */
+typedef enum xg_template {
+    XG_TPL_AFFINE_CLIP = 0,
+    XG_TPL_ROW_SUM = 1,
+    XG_TPL_ROW_SOFTMAX = 2,
+    XG_TPL_BITMIX = 3
+} xg_template;
+
+typedef struct xg_request {
+    int32_t tpl;             /* xg_template                                   */
+    int32_t dtype;
+    int64_t shape[4];        /* AFFINE_CLIP: N, C, H, W; ROW_*: rows, cols;
+                                BITMIX: n                                      */
+    const void* in;
+    const void* params[4];   /* AFFINE_CLIP: mean, var, gamma, beta ([C] each) */
+    void* out;
+    uint64_t tag;            /* returned in the completion                    */
+    int64_t arrival_ns;      /* xg_server_now_ns() time the request arrived;
+                                0 = the time of submission                     */
+} xg_request;
+
+typedef struct xg_server_config {
+    int64_t hbm_budget_bytes; /* operand + output bytes of admitted, unfinished
+                                 requests (a request larger than this is
+                                 rejected at submission)                       */
+    int32_t streams;          /* groups in flight on the device (1..16)        */
+    int32_t max_batch;        /* requests fused into one launch (>= 1)         */
+    int32_t queue_capacity;   /* submitted, not yet admitted; beyond it
+                                 submission returns XG_BUSY                    */
+    int32_t plain;            /* 1: no fusion, each request runs the plain
+                                 per-expression path (the reference sequence
+                                 of assign / reduce launches) on its own      */
+} xg_server_config;
+
+typedef struct xg_completion {
+    uint64_t tag;
+    int32_t status;          /* xg_status of the request                      */
+    int32_t batch_size;      /* requests in the launch that ran it            */
+    int64_t arrival_ns;
+    int64_t admit_ns;
+    int64_t launch_ns;
+    int64_t done_ns;         /* when the loop saw the launch complete         */
+} xg_completion;
+
+typedef struct xg_server_stats {
+    int64_t submitted;
+    int64_t completed;
+    int64_t rejected;
+    int64_t launches;        /* kernel-group launches (fused or plain)        */
+    int64_t inflight_bytes;
+    int64_t peak_inflight_bytes;
+    int64_t queued;          /* submitted, not yet admitted                   */
+    int64_t admitted;        /* admitted, not yet complete                    */
+} xg_server_stats;
+
+typedef struct xg_server_s* xg_server_t;
+
+/* Monotonic clock of the server, nanoseconds. */
+int64_t xg_server_now_ns(void);
+/* Start a server (and its loop thread) on the current device. */
+xg_status xg_server_create(const xg_server_config* cfg, xg_server_t* server);
+/* Queue a request.  XG_BUSY: queue full (retry later); XG_OUT_OF_MEMORY:
+ * the request alone exceeds the budget; XG_TYPE_ERROR / XG_INVALID_ARGUMENT:
+ * malformed request.  Returns without waiting for the device. */
+xg_status xg_server_submit(xg_server_t server, const xg_request* req);
+/* Move up to `cap` completions into `out`; *n receives how many. */
+xg_status xg_server_poll(xg_server_t server, xg_completion* out, int32_t cap, int32_t* n);
+/* Wait until every submitted request has completed (timeout_ms < 0: no limit). */
+xg_status xg_server_drain(xg_server_t server, int64_t timeout_ms);
+xg_status xg_server_stats_get(xg_server_t server, xg_server_stats* stats);
+/* hold = 1: keep admitting requests but start no launches (requests
+ * accumulate into groups); hold = 0: resume. */
+xg_status xg_server_hold(xg_server_t server, int32_t hold);
+/* Drain, stop the loop thread and release the server's resources. */
+void xg_server_destroy(xg_server_t server);
 
 #ifdef __cplusplus
 }

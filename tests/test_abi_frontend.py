@@ -52,6 +52,8 @@ int main(void) {
          sizeof(xg_expr), sizeof(xg_reduce_desc), sizeof(xg_scan_desc));
   printf("%zu %zu %zu\n", offsetof(xg_reduce_desc, initial), offsetof(xg_expr, nodes), offsetof(xg_tensor, shape));
   printf("%zu %zu %zu\n", sizeof(xg_batch_item), sizeof(xg_batch_tuning), offsetof(xg_batch_item, out_row_stride));
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(xg_request), offsetof(xg_request, out), sizeof(xg_server_config),
+         sizeof(xg_completion), offsetof(xg_completion, done_ns), sizeof(xg_server_stats));
   return 0;
 }
 '''
@@ -65,8 +67,23 @@ int main(void) {
     mine = [C.sizeof(A.Tensor), C.sizeof(A.Scalar), C.sizeof(A.Node), C.sizeof(A.Expr),
             C.sizeof(A.ReduceDesc), C.sizeof(A.ScanDesc),
             A.ReduceDesc.initial.offset, A.Expr.nodes.offset, A.Tensor.shape.offset,
-            C.sizeof(A.BatchItem), C.sizeof(A.BatchTuning), A.BatchItem.out_row_stride.offset]
+            C.sizeof(A.BatchItem), C.sizeof(A.BatchTuning), A.BatchItem.out_row_stride.offset,
+            C.sizeof(A.Request), A.Request.out.offset, C.sizeof(A.ServerConfig),
+            C.sizeof(A.Completion), A.Completion.done_ns.offset, C.sizeof(A.ServerStats)]
     assert sizes == mine
+
+
+def test_server_config_errors_without_device():
+    """The configuration is validated before any device call."""
+    lib = A.lib()
+    h = C.c_void_p()
+    assert lib.xg_server_create(None, C.byref(h)) == A.INVALID_ARGUMENT
+    for cfg in (A.ServerConfig(0, 4, 8, 16, 0), A.ServerConfig(1 << 30, 0, 8, 16, 0),
+                A.ServerConfig(1 << 30, 17, 8, 16, 0), A.ServerConfig(1 << 30, 4, 0, 16, 0),
+                A.ServerConfig(1 << 30, 4, 8, 0, 0)):
+        assert lib.xg_server_create(C.byref(cfg), C.byref(h)) == A.INVALID_ARGUMENT
+    assert lib.xg_server_submit(None, None) == A.INVALID_ARGUMENT
+    assert lib.xg_plan_execute_many(None, 2, None) == A.INVALID_ARGUMENT
 
 
 def test_batch_argument_errors_without_device():

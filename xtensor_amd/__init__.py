@@ -19,5 +19,7 @@ from .core import (  # noqa: F401
 )
 
 from .batch import Batch  # noqa: F401,E402
+from .core import execute_many  # noqa: F401,E402
+from .serve import Server  # noqa: F401,E402
 
 __version__ = "0.1.0"

@@ -113,6 +113,27 @@ class BatchTuning(C.Structure):
                 ("dyn_percent", C.c_int32)]
 
 
+class Request(C.Structure):
+    _fields_ = [("tpl", C.c_int32), ("dtype", C.c_int32), ("shape", C.c_int64 * 4), ("in_", C.c_void_p),
+                ("params", C.c_void_p * 4), ("out", C.c_void_p), ("tag", C.c_uint64), ("arrival_ns", C.c_int64)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("streams", C.c_int32), ("max_batch", C.c_int32),
+                ("queue_capacity", C.c_int32), ("plain", C.c_int32)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("status", C.c_int32), ("batch_size", C.c_int32), ("arrival_ns", C.c_int64),
+                ("admit_ns", C.c_int64), ("launch_ns", C.c_int64), ("done_ns", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
+                ("launches", C.c_int64), ("inflight_bytes", C.c_int64), ("peak_inflight_bytes", C.c_int64),
+                ("queued", C.c_int64), ("admitted", C.c_int64)]
+
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libxg.so")
 
@@ -142,6 +163,15 @@ EXPORTS = {
     "xg_batch_execute": (C.c_int, [C.c_void_p, C.c_void_p]),
     "xg_batch_info": (C.c_int, [C.c_void_p, _P(C.c_int64), _P(C.c_int64)]),
     "xg_batch_destroy": (None, [C.c_void_p]),
+    "xg_plan_execute_many": (C.c_int, [_P(C.c_void_p), C.c_int64, C.c_void_p]),
+    "xg_server_now_ns": (C.c_int64, []),
+    "xg_server_create": (C.c_int, [_P(ServerConfig), _P(C.c_void_p)]),
+    "xg_server_submit": (C.c_int, [C.c_void_p, _P(Request)]),
+    "xg_server_poll": (C.c_int, [C.c_void_p, _P(Completion), C.c_int32, _P(C.c_int32)]),
+    "xg_server_drain": (C.c_int, [C.c_void_p, C.c_int64]),
+    "xg_server_stats_get": (C.c_int, [C.c_void_p, _P(ServerStats)]),
+    "xg_server_destroy": (None, [C.c_void_p]),
+    "xg_server_hold": (C.c_int, [C.c_void_p, C.c_int32]),
 }
 
 # diagnostic entry points (not part of include/xg.h)

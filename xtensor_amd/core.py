@@ -860,6 +860,14 @@ def plan(out: DeviceArray, e) -> Plan:
     return Plan(out, e)
 
 
+def execute_many(plans: Sequence[Plan], stream=None) -> None:
+    """Enqueue independent plans; elementwise plans sharing a compiled program
+    and rank run as one fused launch (xg_plan_execute_many)."""
+    arr = (C.c_void_p * max(1, len(plans)))(*[p.handle.value for p in plans])
+    st = _stream() if stream is None else C.c_void_p(stream)
+    _check(A.lib().xg_plan_execute_many(arr, len(plans), st))
+
+
 def last_kernel() -> str:
     """Kernel family the last call on this thread ran (diagnostic)."""
     f = getattr(A.lib(), "xg_last_kernel", None)

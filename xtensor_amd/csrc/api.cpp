@@ -100,6 +100,18 @@ void* workspace(size_t bytes, hipStream_t stream) {
 
 using namespace xg;
 
+// plan with the compiled-kernel vector width; without a matching compiled
+// program, re-plan for the generic interpreter (V = 4)
+EwPlan xg::plan_assign_best(const xg_expr* e, const xg_tensor* out, int* V) {
+    *V = choose_vector_width(e, out);
+    EwPlan p = plan_assign(e, out, *V);
+    if (*V != 4 && !ew_find_spec(p, *V)) {
+        *V = 4;
+        p = plan_assign(e, out, 4);
+    }
+    return p;
+}
+
 extern "C" {
 
 const char* xg_version(void) { return "xtensor_amd 0.1.0 (gfx950)"; }
@@ -122,18 +134,6 @@ xg_status xg_expr_info(const xg_expr* e, int32_t* dtype, int32_t* ndim, int64_t*
         *ndim = int32_t(info.shape.size());
         for (size_t k = 0; k < info.shape.size(); ++k) shape[k] = info.shape[k];
     });
-}
-
-// plan with the compiled-kernel vector width; without a matching compiled
-// program, re-plan for the generic interpreter (V = 4)
-static EwPlan plan_assign_best(const xg_expr* e, const xg_tensor* out, int* V) {
-    *V = choose_vector_width(e, out);
-    EwPlan p = plan_assign(e, out, *V);
-    if (*V != 4 && !ew_find_spec(p, *V)) {
-        *V = 4;
-        p = plan_assign(e, out, 4);
-    }
-    return p;
 }
 
 xg_status xg_assign(const xg_expr* e, const xg_tensor* out, xg_stream_t stream) {
@@ -338,6 +338,34 @@ xg_status xg_plan_execute(xg_plan_t p, void* ws, xg_stream_t stream) {
             check_hip(launch_scan(p->scan, p->V, st, &used), "scan launch");
         }
         note_kernel(used);
+    });
+}
+
+xg_status xg_plan_execute_many(const xg_plan_t* plans, int64_t n, xg_stream_t stream) {
+    return guard([&] {
+        if (n < 0 || (n > 0 && !plans)) raise(XG_INVALID_ARGUMENT, "bad plan list");
+        hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+        std::vector<const EwPlan*> ew;
+        std::vector<int> V;
+        for (int64_t i = 0; i < n; ++i) {
+            if (!plans[i]) raise(XG_INVALID_ARGUMENT, "null plan in list");
+            if (plans[i]->kind == 0) {
+                ew.push_back(&plans[i]->ew);
+                V.push_back(plans[i]->V);
+            }
+        }
+        launch_ew_many(ew.data(), V.data(), ew.size(), st);
+        for (int64_t i = 0; i < n; ++i) {
+            xg_plan_s* p = plans[i];
+            const char* used = nullptr;
+            if (p->kind == 1 && p->red.n_out) {
+                p->red.partials = p->ws ? workspace(p->ws, st) : nullptr;
+                check_hip(launch_reduce(p->red, p->V, st, &used), "reduction launch");
+            } else if (p->kind == 2) {
+                check_hip(launch_scan(p->scan, p->V, st, &used), "scan launch");
+            }
+        }
+        note_kernel(ew.size() > 1 ? "multi" : "");
     });
 }
 

@@ -368,6 +368,67 @@ static BatchKernel batch_kernel_for(int op, int dtype) {
 }
 
 // ---------------------------------------------------------------- host side
+int64_t batch_task_cost(const BItem& it, const BTask& t, int dtype) {
+    // register slots the task occupies (padded rows x padded row length)
+    // plus a fixed cost per row-group iteration
+    const int esz = dtype_size(dtype);
+    const int ne = batch_ne_class(it.cols, it.lpr, esz);
+    const int64_t per = 64 / it.lpr;
+    const int64_t step = per * batch_rows_at_once(ne, esz);
+    const int64_t groups = (t.nrows + per - 1) / per;
+    const int64_t slots = ne ? int64_t(ne) * it.lpr : int64_t(it.cols) * 3;  // streamed rows: 3 passes
+    return groups * std::max<int64_t>(slots, 64) + 512 * ((t.nrows + step - 1) / step);
+}
+
+void batch_split_tasks(const BItem* items, size_t n, int op, int dtype, int task_elems, std::vector<BTask>& tasks) {
+    tasks.clear();
+    const int esz = dtype_size(dtype);
+    for (size_t i = 0; i < n; ++i) {
+        const BItem& it = items[i];
+        if (it.rows == 0) continue;
+        if (it.cols == 0 && op == BO_SOFTMAX) continue;
+        const int64_t step = int64_t(64 / it.lpr) * batch_rows_at_once(batch_ne_class(it.cols, it.lpr, esz), esz);
+        const int64_t want = std::max<int64_t>(1, task_elems / std::max<int64_t>(1, it.cols));
+        const int64_t rpt = ((want + step - 1) / step) * step;
+        for (int64_t r0 = 0; r0 < it.rows; r0 += rpt) {
+            const int64_t k = std::min<int64_t>(rpt, it.rows - r0);
+            tasks.push_back(BTask{int32_t(i), int32_t(r0), int32_t(k), 0});
+        }
+    }
+    if (tasks.size() >= (size_t(1) << 31)) raise(XG_UNSUPPORTED, "batch has too many tasks");
+}
+
+BItem batch_item(const void* in, void* out, int64_t rows, int64_t cols, int64_t in_rs, int64_t out_rs, int op,
+                 int dtype) {
+    BItem d{};
+    d.in = static_cast<const char*>(in);
+    d.out = static_cast<char*>(out);
+    // a single row's strides are never used to step; normalise them
+    d.in_rs = rows > 1 ? in_rs : cols;
+    d.out_rs = rows > 1 ? out_rs : (op == BO_SOFTMAX ? cols : 1);
+    d.rows = int32_t(rows);
+    d.cols = int32_t(cols);
+    d.lpr = lanes_per_row(cols);
+    // A wave addresses the rows of a group through one buffer resource with
+    // 32-bit offsets; rows too far apart fall back to one row per wave, which
+    // leaves every result unchanged (the extra lanes only fold in exact
+    // zeros / lowest()).
+    const int64_t esz = dtype_size(dtype);
+    const int64_t rs = std::max<int64_t>(d.in_rs, op == BO_SOFTMAX ? d.out_rs : 0);
+    if (d.lpr < 64 && (int64_t(64 / d.lpr - 1) * rs + cols) * esz >= (int64_t(1) << 31)) d.lpr = 64;
+    return d;
+}
+
+hipError_t launch_batch_rows(int op, int dtype, const BatchArgs& a, int blocks, int block, hipStream_t stream) {
+    BatchKernel k = batch_kernel_for(op, dtype);
+    if (!k) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, stream, a.counter);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || a.ntasks == 0) return e;
+    hipLaunchKernelGGL(k, dim3(unsigned(blocks)), dim3(unsigned(block)), 0, stream, a);
+    return hipGetLastError();
+}
+
 namespace {
 
 // shape class: op, dtype, log2 of the element count, log2 of the mean row length
@@ -444,18 +505,7 @@ struct xg_batch_s {
     int grid_blocks() const { return std::max(1, cus * tune.blocks_per_cu); }
     int grid_waves() const { return grid_blocks() * (tune.block / 64); }
 
-    // estimated work of a task: register slots it occupies (padded rows x
-    // padded row length) plus a fixed cost per row-group iteration
-    int64_t task_cost(const BTask& t) const {
-        const BItem& it = items[size_t(t.item)];
-        const int esz = dtype_size(dtype);
-        const int ne = batch_ne_class(it.cols, it.lpr, esz);
-        const int64_t per = 64 / it.lpr;
-        const int64_t step = per * batch_rows_at_once(ne, esz);
-        const int64_t groups = (t.nrows + per - 1) / per;
-        const int64_t slots = ne ? int64_t(ne) * it.lpr : int64_t(it.cols) * 3;  // streamed rows: 3 passes
-        return groups * std::max<int64_t>(slots, 64) + 512 * ((t.nrows + step - 1) / step);
-    }
+    int64_t task_cost(const BTask& t) const { return batch_task_cost(items[size_t(t.item)], t, dtype); }
 
     // Split every instance into tasks of about tune.task_elems elements
     // (whole row-group iterations of one wave), then schedule them: all but
@@ -463,21 +513,7 @@ struct xg_batch_s {
     // the least loaded wave of the grid (no atomics, balanced to within one
     // task); the rest, largest first, goes to the shared work queue.
     void build_schedule() {
-        tasks.clear();
-        const int esz = dtype_size(dtype);
-        for (size_t i = 0; i < items.size(); ++i) {
-            const BItem& it = items[i];
-            if (it.rows == 0) continue;
-            if (it.cols == 0 && op == BO_SOFTMAX) continue;
-            const int64_t step = int64_t(64 / it.lpr) * batch_rows_at_once(batch_ne_class(it.cols, it.lpr, esz), esz);
-            const int64_t want = std::max<int64_t>(1, tune.task_elems / std::max<int64_t>(1, it.cols));
-            const int64_t rpt = ((want + step - 1) / step) * step;
-            for (int64_t r0 = 0; r0 < it.rows; r0 += rpt) {
-                const int64_t n = std::min<int64_t>(rpt, it.rows - r0);
-                tasks.push_back(BTask{int32_t(i), int32_t(r0), int32_t(n), 0});
-            }
-        }
-        if (tasks.size() >= (size_t(1) << 31)) raise(XG_UNSUPPORTED, "batch has too many tasks");
+        batch_split_tasks(items.data(), items.size(), op, dtype, tune.task_elems, tasks);
         std::vector<int64_t> cost(tasks.size());
         std::vector<size_t> order(tasks.size());
         int64_t total = 0;
@@ -629,23 +665,7 @@ xg_status xg_batch_create(int32_t op, int32_t dtype, const xg_batch_item* items,
             const xg_batch_item& s = items[i];
             check_item(s, op, i);
             BItem& d = b->items[size_t(i)];
-            d.in = static_cast<const char*>(s.in);
-            d.out = static_cast<char*>(s.out);
-            // a single row's strides are never used to step; normalise them
-            d.in_rs = s.rows > 1 ? s.in_row_stride : s.cols;
-            d.out_rs = s.rows > 1 ? s.out_row_stride : (op == BO_SOFTMAX ? s.cols : 1);
-            d.rows = int32_t(s.rows);
-            d.cols = int32_t(s.cols);
-            d.lpr = lanes_per_row(s.cols);
-            // A wave addresses the rows of a group through one buffer
-            // resource with 32-bit offsets; rows too far apart fall back to
-            // one row per wave, which leaves every result unchanged (the
-            // extra lanes only fold in exact zeros / lowest()).
-            {
-                const int64_t esz = dtype_size(dtype);
-                const int64_t rs = std::max<int64_t>(d.in_rs, op == BO_SOFTMAX ? d.out_rs : 0);
-                if (d.lpr < 64 && (int64_t(64 / d.lpr - 1) * rs + s.cols) * esz >= (int64_t(1) << 31)) d.lpr = 64;
-            }
+            d = batch_item(s.in, s.out, s.rows, s.cols, s.in_row_stride, s.out_row_stride, op, dtype);
             d.pad_ = 0;
             b->elements += s.rows * s.cols;
             rows_total += s.rows;

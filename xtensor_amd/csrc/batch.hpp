@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "xg_internal.hpp"
 
 namespace xg {
@@ -73,5 +75,18 @@ XG_HD constexpr int batch_rows_at_once(int ne, int esz) {
     const int rb = cap / ne;
     return rb < 1 ? 1 : (rb > 8 ? 8 : rb);
 }
+
+// Kernel view of one instance (strides normalised, lanes per row chosen).
+BItem batch_item(const void* in, void* out, int64_t rows, int64_t cols, int64_t in_rs, int64_t out_rs, int op,
+                 int dtype);
+
+// Split instances into tasks of about task_elems elements (whole row-group
+// iterations of one wave), in instance order; and the estimated cost of one.
+void batch_split_tasks(const BItem* items, size_t n, int op, int dtype, int task_elems, std::vector<BTask>& tasks);
+int64_t batch_task_cost(const BItem& it, const BTask& t, int dtype);
+
+// Enqueue the queue reset and the row kernel of (op, dtype): blocks x block
+// threads; a.wbeg must hold blocks * block / 64 + 1 entries.
+hipError_t launch_batch_rows(int op, int dtype, const BatchArgs& a, int blocks, int block, hipStream_t stream);
 
 }  // namespace xg

@@ -92,8 +92,15 @@ __device__ __forceinline__ void load_at(const char* p, int mode, int dt, int32_t
     }
 }
 
-template <class S, int I, int V, int NR, int RANK>
-__device__ __forceinline__ void run_spec(const EwPlan& plan, const uint32_t (&idx)[4], uint32_t j0, int nvalid,
+// Plan accessors: the single-expression kernels read an EwPlan, the fused
+// launches one CompactPlan per expression.
+__device__ __forceinline__ const char* leaf_ptr(const EwPlan& p, int l) { return p.leaves[l].ptr; }
+__device__ __forceinline__ const uint64_t* plan_consts(const EwPlan& p) { return p.prog.consts; }
+__device__ __forceinline__ const char* leaf_ptr(const CompactPlan& p, int l) { return p.leaf[l]; }
+__device__ __forceinline__ const uint64_t* plan_consts(const CompactPlan& p) { return p.consts; }
+
+template <class S, int I, int V, int NR, int RANK, class P>
+__device__ __forceinline__ void run_spec(const P& plan, const uint32_t (&idx)[4], uint32_t j0, int nvalid,
                                          Reg (&R)[NR][V]) {
     if constexpr (I < S::nops) {
         constexpr MicroOp m = S::ops[I];
@@ -101,12 +108,12 @@ __device__ __forceinline__ void run_spec(const EwPlan& plan, const uint32_t (&id
             constexpr int dt = S::leaf_dtype[m.aux];
             constexpr int mode = S::leaf_mode[m.aux];
             const int32_t off = coff<RANK, mode != LM_BCAST>(plan.cst[m.aux], idx, j0);
-            const char* p = plan.leaves[m.aux].ptr + int64_t(off) * dtype_size(dt);
+            const char* p = leaf_ptr(plan, m.aux) + int64_t(off) * dtype_size(dt);
             load_at<V>(p, mode, dt, plan.cst[m.aux][RANK - 1], nvalid, R[m.dst]);
         } else {
             Reg tmp[V];
 #pragma unroll
-            for (int v = 0; v < V; ++v) tmp[v] = exec_scalar(m, R[m.a][v], R[m.b][v], R[m.c][v], plan.prog.consts);
+            for (int v = 0; v < V; ++v) tmp[v] = exec_scalar(m, R[m.a][v], R[m.b][v], R[m.c][v], plan_consts(plan));
 #pragma unroll
             for (int v = 0; v < V; ++v) R[m.dst][v] = tmp[v];
         }
@@ -114,37 +121,60 @@ __device__ __forceinline__ void run_spec(const EwPlan& plan, const uint32_t (&id
     }
 }
 
+// Evaluate work unit `u` (V consecutive elements of one inner row) of a
+// compiled plan and store it.
+template <int V, int NR, class S, int RANK, class P>
+__device__ __forceinline__ void spec_unit(const P& plan, uint32_t u) {
+    const uint32_t row = fastdiv(u, plan.div_chunks);
+    const uint32_t j0 = (u - row * plan.nchunks) * V;
+    uint32_t idx[4] = {0, 0, 0, 0};
+    uint32_t r = row;
+#pragma unroll
+    for (int k = RANK - 2; k >= 1; --k) {
+        const uint32_t q = fastdiv(r, plan.div[k]);
+        idx[k] = r - q * plan.shape32[k];
+        r = q;
+    }
+    if constexpr (RANK >= 2) idx[0] = r;
+    const int nvalid = min(V, int(plan.inner - j0));
+    Reg R[NR][V];
+    run_spec<S, 0, V, NR, RANK>(plan, idx, j0, nvalid, R);
+    constexpr int odt = S::out_dtype;
+    constexpr int esz = dtype_size(odt);
+    const int32_t off = coff<RANK, true>(plan.cst[kMaxLeaves], idx, j0);
+    char* p = plan.out_ptr + int64_t(off) * esz;
+    if (S::out_mode == LM_CONTIG && nvalid == V) {
+        store_vec<V>(p, esz, odt, R[S::result_reg]);
+    } else {
+        const int32_t os = plan.cst[kMaxLeaves][RANK - 1];
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+            if (v < nvalid) store_raw(p + int64_t(v) * os * esz, esz, to_storage(R[S::result_reg][v], odt));
+    }
+}
+
 template <int V, int NR, class S, int RANK>
 __global__ __launch_bounds__(256) void ew_spec_kernel(const EwPlan plan) {
     const uint32_t nunits = uint32_t(plan.units);
     const uint32_t step = gridDim.x * 256u;
-    for (uint32_t u = blockIdx.x * 256u + threadIdx.x; u < nunits; u += step) {
-        const uint32_t row = fastdiv(u, plan.div_chunks);
-        const uint32_t j0 = (u - row * plan.nchunks) * V;
-        uint32_t idx[4] = {0, 0, 0, 0};
-        uint32_t r = row;
-#pragma unroll
-        for (int k = RANK - 2; k >= 1; --k) {
-            const uint32_t q = fastdiv(r, plan.div[k]);
-            idx[k] = r - q * plan.shape32[k];
-            r = q;
+    for (uint32_t u = blockIdx.x * 256u + threadIdx.x; u < nunits; u += step) spec_unit<V, NR, S, RANK>(plan, u);
+}
+
+// Many expressions of one compiled program in one launch: the work units of
+// all plans are numbered consecutively (prefix[i] = first unit of plan i,
+// prefix[n] = total); a thread finds its plan by binary search.
+template <int V, int NR, class S, int RANK>
+__global__ __launch_bounds__(256) void ew_multi_kernel(const CompactPlan* plans, const uint64_t* prefix, uint32_t n) {
+    const uint64_t total = prefix[n];
+    const uint64_t step = uint64_t(gridDim.x) * 256u;
+    for (uint64_t u = uint64_t(blockIdx.x) * 256u + threadIdx.x; u < total; u += step) {
+        uint32_t lo = 0, hi = n;  // prefix[lo] <= u < prefix[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (prefix[mid] <= u) lo = mid;
+            else hi = mid;
         }
-        if constexpr (RANK >= 2) idx[0] = r;
-        const int nvalid = min(V, int(plan.inner - j0));
-        Reg R[NR][V];
-        run_spec<S, 0, V, NR, RANK>(plan, idx, j0, nvalid, R);
-        constexpr int odt = S::out_dtype;
-        constexpr int esz = dtype_size(odt);
-        const int32_t off = coff<RANK, true>(plan.cst[kMaxLeaves], idx, j0);
-        char* p = plan.out_ptr + int64_t(off) * esz;
-        if (S::out_mode == LM_CONTIG && nvalid == V) {
-            store_vec<V>(p, esz, odt, R[S::result_reg]);
-        } else {
-            const int32_t os = plan.cst[kMaxLeaves][RANK - 1];
-#pragma unroll
-            for (int v = 0; v < V; ++v)
-                if (v < nvalid) store_raw(p + int64_t(v) * os * esz, esz, to_storage(R[S::result_reg][v], odt));
-        }
+        spec_unit<V, NR, S, RANK>(plans[lo], uint32_t(u - prefix[lo]));
     }
 }
 
@@ -158,12 +188,14 @@ namespace xg {
 namespace {
 
 using EwFn = void (*)(const EwPlan);
+using EwMultiFn = void (*)(const CompactPlan*, const uint64_t*, uint32_t);
 
 struct SpecEntry {
     const char* name;
     int V;
     SpecKey key;
-    EwFn by_rank[4];  // compiled kernels for normalised rank 1..4
+    EwFn by_rank[4];          // compiled kernels for normalised rank 1..4
+    EwMultiFn multi_by_rank[4];  // fused many-expression kernels, rank 1..4
 };
 
 template <class S> SpecKey key_of() {
@@ -188,7 +220,9 @@ std::vector<SpecEntry>& spec_table() {
 #define XG_REGISTER_SPEC(S)                                                                                   \
     t.push_back(SpecEntry{#S, S::V, key_of<S>(),                                                              \
                           {ew_spec_kernel<S::V, S::nregs, S, 1>, ew_spec_kernel<S::V, S::nregs, S, 2>,          \
-                           ew_spec_kernel<S::V, S::nregs, S, 3>, ew_spec_kernel<S::V, S::nregs, S, 4>}});
+                           ew_spec_kernel<S::V, S::nregs, S, 3>, ew_spec_kernel<S::V, S::nregs, S, 4>},         \
+                          {ew_multi_kernel<S::V, S::nregs, S, 1>, ew_multi_kernel<S::V, S::nregs, S, 2>,        \
+                           ew_multi_kernel<S::V, S::nregs, S, 3>, ew_multi_kernel<S::V, S::nregs, S, 4>}});
         XG_SPEC_LIST(XG_REGISTER_SPEC)
 #undef XG_REGISTER_SPEC
     });
@@ -216,6 +250,48 @@ int ew_grid(uint64_t units, int cus) {
     if (blocks > 0x7fffffffull) blocks = 0x7fffffffull;
     if (blocks < 1) blocks = 1;
     return int(blocks);
+}
+
+int ew_spec_index(const EwPlan& p, int V) {
+    if (g_force_dynamic || !p.compact) return -1;
+    SpecKey k = spec_key(p);
+    auto& t = spec_table();
+    for (size_t i = 0; i < t.size(); ++i)
+        if (t[i].V == V && spec_equal(t[i].key, k)) return int(i);
+    return -1;
+}
+
+const char* ew_spec_name(int index) {
+    auto& t = spec_table();
+    return index >= 0 && size_t(index) < t.size() ? t[size_t(index)].name : nullptr;
+}
+
+CompactPlan make_compact(const EwPlan& p) {
+    CompactPlan c{};
+    for (int k = 0; k < 4; ++k) {
+        c.shape32[k] = p.shape32[k];
+        c.div[k] = p.div[k];
+    }
+    c.div_chunks = p.div_chunks;
+    c.nchunks = p.nchunks;
+    c.inner = p.inner;
+    c.units = p.units;
+    for (int l = 0; l <= kMaxLeaves; ++l)
+        for (int k = 0; k < 4; ++k) c.cst[l][k] = p.cst[l][k];
+    for (int l = 0; l < kMaxLeaves; ++l) c.leaf[l] = p.leaves[l].ptr;
+    c.out_ptr = p.out_ptr;
+    for (int i = 0; i < kMaxConsts; ++i) c.consts[i] = p.prog.consts[i];
+    return c;
+}
+
+hipError_t launch_ew_multi(int spec_index, int rank, const CompactPlan* d_plans, const uint64_t* d_prefix, uint32_t n,
+                           uint64_t total_units, hipStream_t stream) {
+    auto& t = spec_table();
+    if (spec_index < 0 || size_t(spec_index) >= t.size() || rank < 1 || rank > 4 || n == 0) return hipErrorInvalidValue;
+    if (total_units == 0) return hipSuccess;
+    dim3 grid(ew_grid(total_units, 0)), block(256);
+    hipLaunchKernelGGL(t[size_t(spec_index)].multi_by_rank[rank - 1], grid, block, 0, stream, d_plans, d_prefix, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_ew(const EwPlan& p, int V, int cus, hipStream_t stream, const char** used) {

@@ -17,4 +17,30 @@ const char* ew_find_spec(const EwPlan& p, int V);
 // Route every launch through the dynamic interpreter (testing aid).
 void ew_set_force_dynamic(int on);
 
+// One expression of a fused launch: the compact (rank <= 4, 32-bit offset)
+// part of an EwPlan that a compiled program needs at run time.
+struct CompactPlan {
+    uint32_t shape32[4];
+    FastDiv div[4];
+    FastDiv div_chunks;
+    uint32_t nchunks;
+    uint32_t inner;
+    uint64_t units;
+    int32_t cst[kMaxLeaves + 1][4];
+    const char* leaf[kMaxLeaves];
+    char* out_ptr;
+    uint64_t consts[kMaxConsts];
+};
+
+// Index of the compiled program matching the plan (-1: none), and its name.
+int ew_spec_index(const EwPlan& p, int V);
+const char* ew_spec_name(int index);
+CompactPlan make_compact(const EwPlan& p);
+
+// Evaluate n plans of compiled program `spec_index` at normalised rank
+// `rank` in one launch.  d_plans / d_prefix live in device memory;
+// d_prefix[i] is the first work unit of plan i, d_prefix[n] the total.
+hipError_t launch_ew_multi(int spec_index, int rank, const CompactPlan* d_plans, const uint64_t* d_prefix, uint32_t n,
+                           uint64_t total_units, hipStream_t stream);
+
 }  // namespace xg

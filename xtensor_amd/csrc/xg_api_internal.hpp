@@ -18,6 +18,15 @@ void check_hip(hipError_t e, const char* what);
 // per (device, stream) scratch buffer of at least `bytes`
 void* workspace(size_t bytes, hipStream_t stream);
 
+// Plan out = e with the compiled-kernel vector width, falling back to the
+// generic interpreter's width when no compiled program matches.
+EwPlan plan_assign_best(const xg_expr* e, const xg_tensor* out, int* V);
+
+// Enqueue many elementwise plans: plans of one compiled program and rank go
+// out as one fused launch (tables uploaded through the stream's staging
+// buffer), the rest one launch each.  Returns the number of launches.
+int launch_ew_many(const EwPlan* const* plans, const int* V, size_t n, hipStream_t stream);
+
 // Run f, turning thrown errors into a status code and the thread's message.
 template <class F> xg_status guard(F&& f) {
     try {
