@@ -27,6 +27,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
+    factor a plain copy of known size gives), or None if not measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            entry = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    return None if entry is None else int(entry["bytes_per_launch"])
 N, C, H, W = 64, 256, 56, 56
 
 
@@ -165,6 +178,84 @@ def uneven_batch(np, torch, xt, steps=20, n_inst=4096):
         "plain_launches": len(plans), "speedup_vs_plain": round(plain / wall, 1),
         "max_rel_err_vs_fp64": worst, "tuning": tuning, "tasks": b.info()["tasks"],
     }
+
+
+def reductions(np, torch, xt, steps=10):
+    """Axis reductions and the scan on X float32 [8192, 4096] ~ N(1, 1) (seed 2),
+    its bfloat16 copy, its transpose view and an int32 copy U[-1000, 1000];
+    plus the fused broadcast expression of a 2048 x 1024 float64 pair.  Each
+    result: kernel time, GB/s of algorithmic bytes (input read once + output
+    written) and the relative error against float64 (int: exact)."""
+    rng = np.random.default_rng(2)
+    R, Cc = 8192, 4096
+    X = rng.normal(1.0, 1.0, (R, Cc)).astype(np.float32)
+    Xi = rng.integers(-1000, 1001, (R, Cc), dtype=np.int32)
+    dX = xt.from_numpy(X)
+    dXb = xt.from_numpy(X, "bfloat16")
+    dXi = xt.from_numpy(Xi)
+    X64 = X.astype(np.float64)
+    Xb64 = torch.from_numpy(X).to(torch.bfloat16).to(torch.float64).numpy()
+    out = {}
+
+    def run(name, e, ref, in_bytes, exact=False, dtype=None):
+        if isinstance(e, xt.Reducer):
+            dt, shp = xt.reduce_info(e)
+            o = xt.empty(shp, dtype or xt.dtype_name(dt))
+        else:
+            o = xt.empty(xt.expr_info(e)[1], dtype or xt.dtype_name(xt.expr_info(e)[0]))
+        p = xt.plan(o, e)
+        _, kms = time_plans(torch, [p], steps, 2)
+        got = o.to_numpy().astype(np.float64)
+        ref = np.asarray(ref, np.float64)
+        if exact:
+            err = 0.0 if np.array_equal(got, ref) else float("inf")
+        else:
+            err = float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)))
+        nbytes = in_bytes + o.size * o.itemsize
+        out[name] = {"kernel_ms": round(kms, 4), "GBps": round(nbytes / (kms * 1e-3) / 1e9, 1),
+                     "max_rel_err_vs_fp64": err, "kernel": xt.last_kernel()}
+        del o, p
+
+    nb = R * Cc * 4
+    run("sum_axis0_f32", xt.sum(dX, [0]), X64.sum(0), nb)
+    run("sum_axis1_f32", xt.sum(dX, [1]), X64.sum(1), nb)
+    run("mean_axis0_f32", xt.mean(dX, [0]), X64.mean(0), nb)
+    run("mean_axis1_f32", xt.mean(dX, [1]), X64.mean(1), nb)
+    run("amax_all_f32", xt.amax(dX), X64.max(), nb, exact=True)
+    run("norm_l2_all_f32", xt.norm_l2(dX), np.sqrt((X64 * X64).sum()), nb)
+    run("sum_axis0_transposed_f32", xt.sum(dX.T, [0]), X64.sum(1), nb)
+    run("sum_axis1_transposed_f32", xt.sum(dX.T, [1]), X64.sum(0), nb)
+    run("sum_axis0_bf16", xt.sum(dXb, [0]), Xb64.sum(0), nb // 2)
+    run("sum_axis1_bf16", xt.sum(dXb, [1]), Xb64.sum(1), nb // 2)
+    run("sum_int64_axis0_i32", xt.sum(dXi, [0], dtype="int64"), Xi.astype(np.int64).sum(0), nb, exact=True)
+    run("sum_int64_axis1_i32", xt.sum(dXi, [1], dtype="int64"), Xi.astype(np.int64).sum(1), nb, exact=True)
+    # cumsum along axis 1 (a scan plan)
+    cs = xt.empty((R, Cc), "float32")
+    pcs = xt.plan(cs, xt.Scan("cumsum", dX, 1))
+    _, kms = time_plans(torch, [pcs], steps, 2)
+    ref = np.cumsum(X64, axis=1)
+    got = cs.to_numpy().astype(np.float64)
+    out["cumsum_axis1_f32"] = {"kernel_ms": round(kms, 4), "GBps": round(2 * nb / (kms * 1e-3) / 1e9, 1),
+                               "max_abs_err_over_rowsum_abs": float(np.max(np.abs(got - ref) /
+                                                                    np.abs(X64).sum(1, keepdims=True))),
+                               "kernel": xt.last_kernel()}
+    del cs, pcs
+    # the broadcast expression of 2048 x 1024 float64 and its reductions
+    r1 = np.random.default_rng(0)
+    a = r1.uniform(-1, 1, (2048, 1024))
+    b = r1.uniform(-1, 1, 1024)
+    da, db = xt.from_numpy(a), xt.from_numpy(b)
+    c = xt.empty((2048, 1024), "float64")
+    pc = xt.plan(c, da * db + xt.exp(-da * da))
+    _, kms = time_plans(torch, [pc], steps, 2)
+    c_ref = a * b + np.exp(-a * a)
+    out["mul_add_exp_f64_2048x1024"] = {
+        "kernel_ms": round(kms, 4), "GBps": round((2 * 2048 * 1024 + 1024) * 8 / (kms * 1e-3) / 1e9, 1),
+        "max_rel_err_vs_host_libm": float(np.max(np.abs(c.to_numpy() - c_ref) / np.maximum(np.abs(c_ref), 1e-300))),
+        "kernel": xt.last_kernel()}
+    run("sum_axis1_of_c_f64", xt.sum(c, [1]), c_ref.sum(1), 2048 * 1024 * 8)
+    run("amax_axis0_of_c_f64", xt.amax(c, [0]), c.to_numpy().max(0), 2048 * 1024 * 8, exact=True)
+    return out
 
 
 SERVE_TEMPLATES = {  # template -> dtypes it is defined for
@@ -350,7 +441,10 @@ def main():
     _, ikms = time_plans(torch, ip, max(5, args.steps // 5), 2)
     extra["int32_bitmix_GBps"] = round(N * C * H * W * 8 / (ikms * 1e-3) / 1e9, 1)
     del xi, yi, ip
+    extra["reductions_8192x4096"] = reductions(np, torch, xt)
+    print("reductions done", file=sys.stderr, flush=True)
     extra["uneven_batch_softmax_f32"] = uneven_batch(np, torch, xt)
+    print("uneven batch done", file=sys.stderr, flush=True)
     if not args.no_serve:
         extra["served_stream"] = serving(np, torch, xt)
 
@@ -370,7 +464,7 @@ def main():
         "config": {"workload": "affine_clip_f32_strided_view", "x": [N, C, H, W],
                    "view": "[:, :, 0:56:2, :]", "params": [1, C, 1, 1], "algorithmic_bytes": nbytes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel_name)},
         "extra": extra,
     }
     if not args.no_cpu:

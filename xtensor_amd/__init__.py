@@ -8,7 +8,7 @@ ABI of `include/xg.h`; there is no CPU fallback.
 """
 
 from .core import (  # noqa: F401
-    BroadcastError, DeviceArray, DeviceError, Expr, Func, Reducer, Scalar, assign_checked,
+    BroadcastError, DeviceArray, DeviceError, Expr, Func, Reducer, Scalar, Scan, assign_checked,
     abs, acos, all_, amax, amin, asarray, asin, assign, atan, atan2, broadcast, cast, cbrt, ceil, clip,
     cos, cosh, cumprod, cumsum, dtype_code, dtype_name, empty, equal, erf, eval, exp, exp2, expm1,
     expr_info, fabs, floor, fmod, from_numpy, greater, hypot, last_kernel, less, log, log1p, log2,

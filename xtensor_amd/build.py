@@ -112,6 +112,17 @@ def build(force: bool = False, jobs: int | None = None) -> str:
         tmp = LIB + ".tmp"
         _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
         os.replace(tmp, LIB)
+
+    # 3. the C++ front end's test program (host code against libxg.so)
+    cpp_test = os.path.join(ROOT, "tests", "cpp", "test_xdevice.cpp")
+    exe = os.path.join(BUILD, "test_xdevice")
+    cpp_hdr = os.path.join(HERE, "cpp", "xdevice.hpp")
+    if os.path.exists(cpp_test) and (force or not os.path.exists(exe) or
+                                     _newest([cpp_test, cpp_hdr, LIB] + headers) > os.path.getmtime(exe)):
+        _run(["g++", "-std=c++17", "-O1", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", "-I", INCLUDE,
+              "-I", os.path.join(HERE, "cpp"), cpp_test, "-o", exe, "-L", LIBDIR, "-lxg",
+              "-L", "/opt/rocm/lib", "-lamdhip64", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN/../lib",
+              "-Wl,-rpath,/opt/rocm/lib"])
     return LIB
 
 

@@ -787,6 +787,25 @@ def eval(e) -> DeviceArray:  # noqa: A001
 
 # ----------------------------------------------------------------------------
 # accumulators (reducers/xaccumulator.hpp) — immediate, like xtensor's
+class Scan:
+    """A scan to prepare with `plan(out, Scan(...))`: cumsum / cumprod of e
+    along `axis` (a plan needs an axis; the flattened form runs through
+    cumsum(e) directly)."""
+
+    def __init__(self, kind, e, axis, dtype=None):
+        self.kind = kind
+        self.expr = _wrap(e)
+        self.axis = int(axis)
+        self.dtype = dtype
+
+    def to_abi(self) -> A.ScanDesc:
+        d = A.ScanDesc()
+        d.op = A.SCAN[self.kind]
+        d.acc_dtype = -1 if self.dtype is None else dtype_code(self.dtype)
+        d.axis = self.axis
+        return d
+
+
 def _scan(kind, e, axis=None, dtype=None) -> DeviceArray:
     e = _wrap(e)
     d = A.ScanDesc()
@@ -823,11 +842,17 @@ class Plan:
     """
 
     def __init__(self, out: DeviceArray, e):
-        e = _wrap(e)
+        e = e if isinstance(e, Scan) else _wrap(e)
         self._keep = [out, e]
         h = C.c_void_p()
         L = A.lib()
-        if isinstance(e, Reducer):
+        if isinstance(e, Scan):
+            _require_device(_leaves(e.expr) + [out])
+            s = serialize(e.expr, reducer_fn=_eval_reducer)
+            d = e.to_abi()
+            t = out.to_abi()
+            _check(L.xg_plan_scan(s.ptr(), C.byref(d), C.byref(t), C.byref(h)))
+        elif isinstance(e, Reducer):
             _require_device(_leaves(e.expr) + [out])
             s = serialize(e.expr, reducer_fn=_eval_reducer)
             d = e.to_abi()
