@@ -39,6 +39,7 @@ template <int OP> struct Acc<AK_F32C, OP> {
     }
     __device__ Acc shfl_xor(int m) const { Acc r; r.s = __shfl_xor(s, m); r.c = __shfl_xor(c, m); return r; }
     __device__ Acc shfl_up(int d) const { Acc r; r.s = __shfl_up(s, d); r.c = __shfl_up(c, d); return r; }
+    __device__ Acc shfl_idx(int src) const { Acc r; r.s = __shfl(s, src); r.c = __shfl(c, src); return r; }
     __device__ Store store() const { return Store{__float_as_uint(s), __float_as_uint(c)}; }
     __device__ void load(const Store& st) { s = __uint_as_float(uint32_t(st.a)); c = __uint_as_float(uint32_t(st.b)); }
     __device__ Reg value(int vdt) const {
@@ -66,6 +67,7 @@ template <int OP> struct Acc<AK_F64C, OP> {
     }
     __device__ Acc shfl_xor(int m) const { Acc r; r.s = __shfl_xor(s, m); r.c = __shfl_xor(c, m); return r; }
     __device__ Acc shfl_up(int d) const { Acc r; r.s = __shfl_up(s, d); r.c = __shfl_up(c, d); return r; }
+    __device__ Acc shfl_idx(int src) const { Acc r; r.s = __shfl(s, src); r.c = __shfl(c, src); return r; }
     __device__ Store store() const { return Store{uint64_t(__double_as_longlong(s)), uint64_t(__double_as_longlong(c))}; }
     __device__ void load(const Store& st) { s = __longlong_as_double(int64_t(st.a)); c = __longlong_as_double(int64_t(st.b)); }
     __device__ Reg value(int) const { return from_f64(s + c); }
@@ -114,6 +116,7 @@ template <int AK, int OP> struct Acc {
     __device__ void merge(const Acc& o) { v = combine(v, o.v); }
     __device__ Acc shfl_xor(int m) const { Acc r; r.v = __shfl_xor(v, m); return r; }
     __device__ Acc shfl_up(int d) const { Acc r; r.v = __shfl_up(v, d); return r; }
+    __device__ Acc shfl_idx(int src) const { Acc r; r.v = __shfl(v, src); return r; }
     __device__ Store store() const { return Store{uint64_t(reg_put<T>(v)), 0}; }
     __device__ void load(const Store& st) { v = reg_get<T>(st.a); }
     __device__ Reg value(int vdt) const { return cvt(reg_put<T>(v), AccType<AK>::dt, vdt); }

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "chunk.hpp"
+#include "spec_eval.hpp"
 #include "ew.hpp"
 #include "planner.hpp"
 
@@ -67,59 +68,6 @@ __global__ __launch_bounds__(256) void ew_kernel(const EwPlan plan) {
 // and 32-bit offsets.  Every loop is unrolled at compile time, leaf types
 // and access modes are constants, so each leaf read is one address
 // computation and one (vector) load.
-
-template <int RANK, bool WITH_J>
-__device__ __forceinline__ int32_t coff(const int32_t (&st)[4], const uint32_t (&idx)[4], uint32_t j0) {
-    int32_t off = WITH_J ? int32_t(j0) * st[RANK - 1] : 0;
-#pragma unroll
-    for (int k = 0; k < RANK - 1; ++k) off += int32_t(idx[k]) * st[k];
-    return off;
-}
-
-template <int V>
-__device__ __forceinline__ void load_at(const char* p, int mode, int dt, int32_t is, int nvalid, Reg (&out)[V]) {
-    const int esz = dtype_size(dt);
-    if (mode == LM_BCAST) {
-        Reg x = from_storage(load_raw(p, esz), dt);
-#pragma unroll
-        for (int v = 0; v < V; ++v) out[v] = x;
-    } else if (mode == LM_CONTIG && nvalid == V) {
-        load_vec<V>(p, esz, dt, out);
-    } else {
-#pragma unroll
-        for (int v = 0; v < V; ++v)
-            out[v] = v < nvalid ? from_storage(load_raw(p + int64_t(v) * is * esz, esz), dt) : 0;
-    }
-}
-
-// Plan accessors: the single-expression kernels read an EwPlan, the fused
-// launches one CompactPlan per expression.
-__device__ __forceinline__ const char* leaf_ptr(const EwPlan& p, int l) { return p.leaves[l].ptr; }
-__device__ __forceinline__ const uint64_t* plan_consts(const EwPlan& p) { return p.prog.consts; }
-__device__ __forceinline__ const char* leaf_ptr(const CompactPlan& p, int l) { return p.leaf[l]; }
-__device__ __forceinline__ const uint64_t* plan_consts(const CompactPlan& p) { return p.consts; }
-
-template <class S, int I, int V, int NR, int RANK, class P>
-__device__ __forceinline__ void run_spec(const P& plan, const uint32_t (&idx)[4], uint32_t j0, int nvalid,
-                                         Reg (&R)[NR][V]) {
-    if constexpr (I < S::nops) {
-        constexpr MicroOp m = S::ops[I];
-        if constexpr (m.op == MO_LOAD) {
-            constexpr int dt = S::leaf_dtype[m.aux];
-            constexpr int mode = S::leaf_mode[m.aux];
-            const int32_t off = coff<RANK, mode != LM_BCAST>(plan.cst[m.aux], idx, j0);
-            const char* p = leaf_ptr(plan, m.aux) + int64_t(off) * dtype_size(dt);
-            load_at<V>(p, mode, dt, plan.cst[m.aux][RANK - 1], nvalid, R[m.dst]);
-        } else {
-            Reg tmp[V];
-#pragma unroll
-            for (int v = 0; v < V; ++v) tmp[v] = exec_scalar(m, R[m.a][v], R[m.b][v], R[m.c][v], plan_consts(plan));
-#pragma unroll
-            for (int v = 0; v < V; ++v) R[m.dst][v] = tmp[v];
-        }
-        run_spec<S, I + 1, V, NR, RANK>(plan, idx, j0, nvalid, R);
-    }
-}
 
 // Evaluate work unit `u` (V consecutive elements of one inner row) of a
 // compiled plan and store it.

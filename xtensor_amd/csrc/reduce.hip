@@ -78,6 +78,18 @@ const RSpecEntry* find_rspec(const RedPlan& p, int V) {
     return nullptr;
 }
 
+// Combine split partials: a thread per output when there are many outputs,
+// a block per output when few outputs carry many partials each.
+static hipError_t launch_combine(const RedPlan& q, hipStream_t stream) {
+    const bool wide = q.splits >= 64 && q.n_out < 4096;
+    KFn comb = combine_kernel(q.acc, q.op, wide);
+    if (!comb) return hipErrorInvalidValue;
+    const uint64_t blocks = wide ? std::min<uint64_t>(q.n_out, 1u << 20)
+                                 : std::min<uint64_t>((q.n_out + 255) / 256, 1u << 20);
+    hipLaunchKernelGGL(comb, dim3(uint32_t(std::max<uint64_t>(blocks, 1))), dim3(256), 0, stream, q);
+    return hipGetLastError();
+}
+
 hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** used) {
     if (p.n_out == 0) return hipSuccess;
     RedKernels ks{};
@@ -93,6 +105,29 @@ hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** use
         if (used) *used = "dynamic";
     }
     dim3 block(256);
+    if (sp && p.compact && (p.layout == RL_ROWS ? (p.rank == 1 ? ks.rows_fast1 : ks.rows_fast2) : ks.cols_fast)) {
+        // compiled fast kernels with their own split of the work
+        RedPlan q = p;
+        q.group = p.f_group;
+        q.splits = p.f_splits;
+        q.per_split = p.f_per_split;
+        q.div_splits = make_fastdiv(uint32_t(p.f_splits));
+        if (q.layout == RL_ROWS) {
+            const uint64_t items = uint64_t(q.n_out) * uint64_t(q.splits);
+            const uint64_t per_block = 4 * (64 / uint64_t(q.group));
+            const uint64_t blocks = (items + per_block - 1) / per_block;
+            if (blocks >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
+            hipLaunchKernelGGL(q.rank == 1 ? ks.rows_fast1 : ks.rows_fast2, dim3(uint32_t(blocks)), block, 0, stream, q);
+        } else {
+            const uint64_t tiles = (q.nchunks_f + 63) / 64;
+            const uint64_t blocks = tiles * uint64_t(q.splits);
+            if (blocks >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
+            hipLaunchKernelGGL(ks.cols_fast, dim3(uint32_t(blocks)), block, 0, stream, q);
+        }
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess || q.splits == 1) return err;
+        return launch_combine(q, stream);
+    }
     if (p.layout == RL_ROWS) {
         uint64_t items = uint64_t(p.n_out) * p.splits;
         uint64_t per_block = 256 / p.group;
@@ -110,11 +145,7 @@ hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** use
     }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess || p.splits == 1) return err;
-    KFn comb = combine_kernel(p.acc, p.op);
-    if (!comb) return hipErrorInvalidValue;
-    uint64_t blocks = std::min<uint64_t>((p.n_out + 255) / 256, 1u << 20);
-    hipLaunchKernelGGL(comb, dim3(uint32_t(std::max<uint64_t>(blocks, 1))), block, 0, stream, p);
-    return hipGetLastError();
+    return launch_combine(p, stream);
 }
 
 }  // namespace xg

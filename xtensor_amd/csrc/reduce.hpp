@@ -67,6 +67,14 @@ struct RedPlan {
     void* partials;     // workspace: splits * n_out accumulators of 16 bytes
     LeafView leaves[kMaxLeaves];
     Program prog;       // element value mapped and converted to the accumulation type
+    // Compact form for the compiled fast kernels: normalised rank <= 2 and
+    // 32-bit offsets (rows: [outputs | fast] or [fast]; cols: [rows | fast]).
+    // cst[l][k] = stride of leaf l along dim k.  f_* = their work split.
+    int32_t compact;
+    int32_t f_group;      // rows: lanes per output (1..64, power of two)
+    int32_t f_splits;     // partial results per output
+    uint32_t f_per_split; // rows: chunks per split; cols: rows per split
+    int32_t cst[kMaxLeaves + 1][4];
 };
 
 // Plan out = reduce(e).  Throws Error.  `V` is the vector width.

@@ -3,8 +3,9 @@
 
 namespace xg {
 
-KFn combine_kernel(int ak, int op) {
-#define XG_C(AK, OP) if (ak == AK && op == OP) return reduce_combine_kernel<AK, OP>;
+KFn combine_kernel(int ak, int op, bool wide) {
+#define XG_C(AK, OP) \
+    if (ak == AK && op == OP) return wide ? reduce_combine_wide_kernel<AK, OP> : reduce_combine_kernel<AK, OP>;
     XG_C(AK_F32C, RO_ADD)
     XG_C(AK_F64C, RO_ADD)
     XG_C(AK_F32, RO_MUL)

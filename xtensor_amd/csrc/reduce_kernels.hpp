@@ -7,6 +7,7 @@
 
 #include "acc.hpp"
 #include "chunk.hpp"
+#include "spec_eval.hpp"
 #include "reduce.hpp"
 
 namespace xg {
@@ -236,6 +237,7 @@ __global__ __launch_bounds__(256) void reduce_cols_kernel(const RedPlan p) {
 }
 
 // ---------------------------------------------------------------- split combine
+// a thread per output: partials merged in split order, loaded 8 at a time
 template <int AK, int OP>
 __global__ __launch_bounds__(256) void reduce_combine_kernel(const RedPlan p) {
     using A = Acc<AK, OP>;
@@ -243,12 +245,173 @@ __global__ __launch_bounds__(256) void reduce_combine_kernel(const RedPlan p) {
         const Store* part = reinterpret_cast<const Store*>(p.partials) + size_t(o) * p.splits;
         A a;
         a.load(part[0]);
-        for (int s = 1; s < p.splits; ++s) {
+        int s = 1;
+        for (; s + 8 <= p.splits; s += 8) {
+            Store st[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) st[k] = part[s + k];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                A b;
+                b.load(st[k]);
+                a.merge(b);
+            }
+        }
+        for (; s < p.splits; ++s) {
             A b;
             b.load(part[s]);
             a.merge(b);
         }
         finish(p, a, o);
+    }
+}
+
+// a block per output, for many partials: thread t merges partials
+// t, t + 256, ... in order, then the block combines in a fixed tree
+template <int AK, int OP>
+__global__ __launch_bounds__(256) void reduce_combine_wide_kernel(const RedPlan p) {
+    using A = Acc<AK, OP>;
+    __shared__ Store lds[4];
+    for (uint32_t o = blockIdx.x; o < p.n_out; o += gridDim.x) {
+        const Store* part = reinterpret_cast<const Store*>(p.partials) + size_t(o) * p.splits;
+        A a;
+        a.init(p.identity);
+        if (int(threadIdx.x) < p.splits) a.load(part[threadIdx.x]);
+        for (int s = int(threadIdx.x) + 256; s < p.splits; s += 256) {
+            A b;
+            b.load(part[s]);
+            a.merge(b);
+        }
+        a = group_combine(a, 256, lds);
+        if (threadIdx.x == 0) finish(p, a, o);
+    }
+}
+
+// ---------------------------------------------------------------- compiled fast kernels
+// For compact plans (normalised rank <= 2, 32-bit offsets) and a compiled
+// program: leaf offsets are one multiply-add per dimension, U chunks (rows)
+// are in flight per lane, and the rows kernel combines inside a wave only
+// (no LDS, no barrier).  p.group / p.splits / p.per_split hold the fast
+// split (f_group / f_splits / f_per_split of the plan).
+
+template <int NR, int V> constexpr int fast_unroll() { return NR * V <= 8 ? 4 : (NR * V <= 16 ? 2 : 1); }
+
+template <class A> __device__ __forceinline__ A wave_group_combine(A a, int G) {
+    const int lane = threadIdx.x & 63;
+    for (int m = 1; m < G; m <<= 1) {
+        A o = a.shfl_xor(m);
+        if (lane & m) {  // lower lane's value first
+            o.merge(a);
+            a = o;
+        } else {
+            a.merge(o);
+        }
+    }
+    return a;
+}
+
+// rows: dims [outputs | fast] (RANK 2) or [fast] (RANK 1, one output);
+// G lanes per (output, split), 64 / G of them per wave
+template <int V, int NR, class S, int AK, int OP, int RANK>
+__global__ __launch_bounds__(256) void reduce_rows_fast_kernel(const RedPlan p) {
+    using A = Acc<AK, OP>;
+    constexpr int U = fast_unroll<NR, V>();
+    const int G = p.group;
+    const int lane = threadIdx.x & 63;
+    const uint32_t per_wave = 64u / uint32_t(G);
+    const uint32_t wi = (blockIdx.x * 4u + uint32_t(threadIdx.x >> 6)) * per_wave + uint32_t(lane) / uint32_t(G);
+    const uint32_t gl = uint32_t(lane) & uint32_t(G - 1);
+    const uint32_t items = p.n_out * uint32_t(p.splits);
+    A acc;
+    acc.init(p.identity);
+    uint32_t o = 0;
+    if (wi < items) {
+        o = fastdiv(wi, p.div_splits);
+        const uint32_t sp = wi - o * uint32_t(p.splits);
+        const uint32_t c0 = sp * p.per_split;
+        const uint32_t c1 = min(p.nchunks_f, c0 + p.per_split);
+        const uint32_t idx[4] = {RANK == 2 ? o : 0u, 0u, 0u, 0u};
+        for (uint32_t c = c0 + gl; c < c1; c += uint32_t(G) * U) {
+            Reg R[U][NR][V];
+            int nv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t cc = c + uint32_t(u * G);
+                nv[u] = cc < c1 ? min(V, int(p.lf - cc * V)) : 0;
+                if (nv[u] > 0) run_spec<S, 0, V, NR, RANK>(p, idx, cc * V, nv[u], R[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int v = 0; v < V; ++v)
+                    if (v < nv[u]) acc.add(R[u][S::result_reg][v]);
+        }
+    }
+    acc = wave_group_combine(acc, G);
+    if (wi < items && gl == 0) {
+        if (p.splits == 1) finish(p, acc, o);
+        else reinterpret_cast<Store*>(p.partials)[wi] = acc.store();
+    }
+}
+
+// cols: dims [rows | fast outputs]; a block = one tile of 64 * V outputs and
+// one split of the rows, its four waves interleaving rows, combined in LDS
+template <int V, int NR, class S, int AK, int OP>
+__global__ __launch_bounds__(256) void reduce_cols_fast_kernel(const RedPlan p) {
+    using A = Acc<AK, OP>;
+    constexpr int U = fast_unroll<NR, V>();
+    __shared__ Store lds[4][64][V];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const uint32_t tile = fastdiv(blockIdx.x, p.div_splits);
+    const uint32_t sp = blockIdx.x - tile * uint32_t(p.splits);
+    const uint32_t cc = tile * 64 + uint32_t(lane);
+    const bool active = cc < p.nchunks_f;
+    const uint32_t j0 = cc * V;
+    const int nvalid = active ? min(V, int(p.lf - j0)) : 0;
+    A acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v].init(p.identity);
+    if (active) {
+        const uint32_t r0 = sp * p.per_split;
+        const uint32_t r1 = min(p.red_count, r0 + p.per_split);
+        for (uint32_t r = r0 + uint32_t(wave); r < r1; r += 4 * U) {
+            Reg R[U][NR][V];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t rr = r + uint32_t(4 * u);
+                ok[u] = rr < r1;
+                const uint32_t idx[4] = {rr, 0u, 0u, 0u};
+                if (ok[u]) run_spec<S, 0, V, NR, 2>(p, idx, j0, nvalid, R[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (ok[u])
+#pragma unroll
+                    for (int v = 0; v < V; ++v)
+                        if (v < nvalid) acc[v].add(R[u][S::result_reg][v]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) lds[wave][lane][v] = acc[v].store();
+    __syncthreads();
+    if (wave == 0 && active) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            if (v < nvalid) {
+                A a;
+                a.load(lds[0][lane][v]);
+                for (int w = 1; w < 4; ++w) {
+                    A b;
+                    b.load(lds[w][lane][v]);
+                    a.merge(b);
+                }
+                const uint32_t o = j0 + uint32_t(v);
+                if (p.splits == 1) finish(p, a, o);
+                else reinterpret_cast<Store*>(p.partials)[size_t(o) * p.splits + sp] = a.store();
+            }
+        }
     }
 }
 
@@ -260,12 +423,15 @@ using KFn = void (*)(const RedPlan);
 
 struct RedKernels {
     KFn rows32, rows64, cols32, cols64;
+    KFn rows_fast1, rows_fast2, cols_fast;  // compiled fast kernels (compact plans), or null
 };
 
 // specialised programs: unrolled x4, 32-bit indexing only (64-bit plans run the generic kernels)
 template <int V, int NR, class S, int AK, int OP> RedKernels spec_kernels_for() {
     return RedKernels{reduce_rows_kernel<V, NR, S, false, AK, OP, 4>, nullptr,
-                      reduce_cols_kernel<V, NR, S, false, AK, OP, 4>, nullptr};
+                      reduce_cols_kernel<V, NR, S, false, AK, OP, 4>, nullptr,
+                      reduce_rows_fast_kernel<V, NR, S, AK, OP, 1>, reduce_rows_fast_kernel<V, NR, S, AK, OP, 2>,
+                      reduce_cols_fast_kernel<V, NR, S, AK, OP>};
 }
 
 // generic interpreter: V = 4, no unrolling
@@ -273,13 +439,15 @@ template <int AK, int OP> RedKernels dyn_kernels_for() {
     return RedKernels{reduce_rows_kernel<4, kMaxRegs, DynSpec, false, AK, OP, 1>,
                       reduce_rows_kernel<4, kMaxRegs, DynSpec, true, AK, OP, 1>,
                       reduce_cols_kernel<4, kMaxRegs, DynSpec, false, AK, OP, 1>,
-                      reduce_cols_kernel<4, kMaxRegs, DynSpec, true, AK, OP, 1>};
+                      reduce_cols_kernel<4, kMaxRegs, DynSpec, true, AK, OP, 1>,
+                      nullptr, nullptr, nullptr};
 }
 
 // generic kernels per accumulator family (defined in reduce_dyn_*.hip)
 bool dyn_reduce_float(int ak, int op, RedKernels* k);
 bool dyn_reduce_int32(int ak, int op, RedKernels* k);
 bool dyn_reduce_int64(int ak, int op, RedKernels* k);
-KFn combine_kernel(int ak, int op);
+// split combine: a thread per output, or (wide) a block per output
+KFn combine_kernel(int ak, int op, bool wide);
 
 }  // namespace xg

@@ -474,11 +474,51 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
     if (p.layout == RL_ROWS) p.per_split = uint32_t((uint64_t(p.red_count) + p.splits - 1) / p.splits);
     else p.per_split = uint32_t((uint64_t(p.red_count) + p.splits - 1) / p.splits);
     p.div_splits = make_fastdiv(uint32_t(p.splits));
+
+    // compact rank <= 2 form and the work split of the compiled fast kernels
+    const bool rows_ok = p.layout == RL_ROWS && p.rank <= 2 && p.n_red == 1;
+    const bool cols_ok = p.layout == RL_COLS && p.rank == 2 && p.n_kept == 0;
+    if (!p.index64 && (rows_ok || cols_ok) && !empty_red) {
+        p.compact = 1;
+        for (int pl = 0; pl < nl; ++pl)
+            for (int k = 0; k < p.rank; ++k) p.cst[pl][k] = int32_t(p.leaves[pl].stride[k]);
+        const uint64_t target_waves = uint64_t(cus) * 16;  // enough loads in flight to cover HBM latency
+        constexpr uint64_t kU = 4;                          // chunks (rows) in flight per lane
+        if (p.layout == RL_ROWS) {
+            const uint64_t chunks = p.nchunks_f;
+            uint64_t g = pow2_ceil(uint32_t(std::max<uint64_t>(1, (chunks + 2 * kU - 1) / (2 * kU))));
+            if (uint64_t(n_out) < target_waves) g = 64;  // few outputs: whole waves, then splits
+            p.f_group = int(std::min<uint64_t>(std::max<uint64_t>(g, 1), 64));
+            const uint64_t waves = (uint64_t(n_out) * p.f_group + 63) / 64;
+            uint64_t s = 1;
+            if (waves < target_waves) {
+                s = (target_waves + waves - 1) / waves;
+                s = std::min<uint64_t>(s, std::max<uint64_t>(1, chunks / (uint64_t(p.f_group) * kU * 2)));
+            }
+            p.f_splits = int(std::min<uint64_t>(s, 65535));
+            p.f_per_split = uint32_t((chunks + p.f_splits - 1) / p.f_splits);
+        } else {
+            const uint64_t rows = p.red_count;
+            const uint64_t tiles = (p.nchunks_f + 63) / 64;
+            const uint64_t target_blocks = target_waves / 4;
+            uint64_t s = (target_blocks + tiles - 1) / tiles;
+            s = std::min<uint64_t>(s, std::max<uint64_t>(1, rows / (4 * kU * 2)));
+            // partial results stay within a few percent of the input bytes
+            s = std::min<uint64_t>(s, std::max<uint64_t>(1, uint64_t(best) / (uint64_t(n_out) * 16 * 32)));
+            p.f_group = 256;
+            p.f_splits = int(std::min<uint64_t>(s, 65535));
+            p.f_per_split = uint32_t((rows + p.f_splits - 1) / p.f_splits);
+        }
+        while (p.f_splits > 1 && uint64_t(n_out) * uint64_t(p.f_splits) >= (uint64_t(1) << 31)) p.f_splits /= 2;
+        if (p.layout == RL_ROWS) p.f_per_split = uint32_t((uint64_t(p.nchunks_f) + p.f_splits - 1) / p.f_splits);
+        else p.f_per_split = uint32_t((uint64_t(p.red_count) + p.f_splits - 1) / p.f_splits);
+    }
     return p;
 }
 
 size_t reduce_workspace_bytes(const RedPlan& p) {
-    return p.splits > 1 ? size_t(p.splits) * size_t(p.n_out) * 16 : 0;
+    const int s = std::max(p.splits, p.compact ? p.f_splits : 1);
+    return s > 1 ? size_t(s) * size_t(p.n_out) * 16 : 0;
 }
 
 }  // namespace xg

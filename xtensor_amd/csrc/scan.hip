@@ -12,12 +12,37 @@
 
 namespace xg {
 
+// The direct rows kernel applies: rank <= 2 rows layout with 32-bit offsets,
+// a program that is one load of a leaf of the accumulation input type, the
+// leaf and the output contiguous along the axis, rows starting 16-byte
+// aligned, output elements of the same size, not bf16.
+static bool scan_direct_ok(const ScanPlan& p) {
+    static const int in_dt[AK_NUM] = {XG_FLOAT32, XG_FLOAT64, XG_FLOAT32, XG_FLOAT64,
+                                      XG_INT32,   XG_UINT32,  XG_INT64,   XG_UINT64};
+    if (p.layout != SL_ROWS || p.index64 || p.rank < 1 || p.rank > 2 || p.acc < 0 || p.acc >= AK_NUM) return false;
+    if (p.rank == 1 && p.rows != 1) return false;
+    if (p.prog.nops != 1 || p.prog.ops[0].op != MO_LOAD || p.prog.nleaves != 1) return false;
+    const LeafView& l = p.leaves[0];
+    const int esz = dtype_size(l.dtype);
+    if (l.dtype != in_dt[p.acc] || l.mode != LM_CONTIG || l.stride[p.rank - 1] != 1) return false;
+    if (p.out_dtype == XG_BFLOAT16 || dtype_size(p.out_dtype) != esz || p.out_stride[p.rank - 1] != 1) return false;
+    if (reinterpret_cast<uintptr_t>(l.ptr) % 16 || reinterpret_cast<uintptr_t>(p.out_ptr) % 16) return false;
+    if (p.rank == 2 && ((l.stride[0] * esz) % 16 || (p.out_stride[0] * esz) % 16)) return false;
+    return true;
+}
+
 hipError_t launch_scan(const ScanPlan& p, int V, hipStream_t stream, const char** used) {
     if (p.rows == 0) return hipSuccess;
     if (V != 4) return hipErrorInvalidValue;
     ScanKernels k{};
     if (!scan_kernels_float(p.acc, p.op, &k) && !scan_kernels_int(p.acc, p.op, &k)) return hipErrorInvalidValue;
     dim3 block(256);
+    if (k.rows_direct && scan_direct_ok(p)) {
+        const uint64_t blocks = (uint64_t(p.rows) + 3) / 4;
+        if (used) *used = "scan_rows_direct";
+        hipLaunchKernelGGL(k.rows_direct, dim3(uint32_t(std::min<uint64_t>(blocks, 0x7fffffff))), block, 0, stream, p);
+        return hipGetLastError();
+    }
     if (p.layout == SL_ROWS) {
         uint32_t blocks = std::min<uint32_t>(p.rows, 1u << 20);
         if (used) *used = "scan_rows";

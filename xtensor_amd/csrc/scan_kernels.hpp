@@ -141,17 +141,122 @@ __global__ __launch_bounds__(256) void scan_cols_kernel(const ScanPlan p) {
     }
 }
 
+template <class T> struct ScanInDtype;
+template <> struct ScanInDtype<float> { static constexpr int value = XG_FLOAT32; };
+template <> struct ScanInDtype<double> { static constexpr int value = XG_FLOAT64; };
+template <> struct ScanInDtype<int32_t> { static constexpr int value = XG_INT32; };
+template <> struct ScanInDtype<uint32_t> { static constexpr int value = XG_UINT32; };
+template <> struct ScanInDtype<int64_t> { static constexpr int value = XG_INT64; };
+template <> struct ScanInDtype<uint64_t> { static constexpr int value = XG_UINT64; };
+
+// element type the direct kernel reads for an accumulator kind
+template <int AK> struct ScanIn;
+template <> struct ScanIn<AK_F32C> { using T = float; };
+template <> struct ScanIn<AK_F64C> { using T = double; };
+template <> struct ScanIn<AK_F32> { using T = float; };
+template <> struct ScanIn<AK_F64> { using T = double; };
+template <> struct ScanIn<AK_I32> { using T = int32_t; };
+template <> struct ScanIn<AK_U32> { using T = uint32_t; };
+template <> struct ScanIn<AK_I64> { using T = int64_t; };
+template <> struct ScanIn<AK_U64> { using T = uint64_t; };
+
+// Direct rows scan for a program that only loads its one leaf (already of
+// the accumulation input type), contiguous and 16-byte aligned along the
+// axis, with a contiguous aligned output: one wave per row (rank 2,
+// [rows | axis]).  The row goes by in segments of 64 * V elements, U of
+// them loaded at once; per segment each lane scans its V elements, a wave
+// scan (shuffles, earlier lanes first) gives the exclusive prefix, and the
+// segment total carries on.  The merge order depends on the row length
+// only.
+template <int AK, int OP>
+__global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p) {
+    using A = Acc<AK, OP>;
+    using T = typename ScanIn<AK>::T;
+    constexpr int V = sizeof(T) == 4 ? 4 : 2;
+    constexpr int U = 4;
+    const int lane = threadIdx.x & 63;
+    const uint32_t row = blockIdx.x * 4u + uint32_t(threadIdx.x >> 6);
+    if (row >= p.rows) return;  // whole waves leave: row is uniform in a wave
+    const T* in = reinterpret_cast<const T*>(p.leaves[0].ptr) + int64_t(row) * p.leaves[0].stride[0];
+    const int odt = p.out_dtype;
+    const int oesz = dtype_size(odt);
+    char* out = p.out_ptr + int64_t(row) * p.out_stride[0] * oesz;
+    A carry;
+    carry.init(p.identity);
+    for (uint32_t base = 0; base < p.len; base += 64u * V * U) {
+        T x[U][V];
+        int nv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t j = base + uint32_t(u * 64 * V) + uint32_t(lane * V);
+            nv[u] = j < p.len ? min(V, int(p.len - j)) : 0;
+            if (nv[u] == V) {
+                Reg r[V];
+                load_vec<V>(reinterpret_cast<const char*>(in + j), int(sizeof(T)), ScanInDtype<T>::value, r);
+#pragma unroll
+                for (int v = 0; v < V; ++v) x[u][v] = reg_get<T>(r[v]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v) x[u][v] = v < nv[u] ? in[j + uint32_t(v)] : T(0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (base + uint32_t(u * 64 * V) >= p.len) break;  // uniform
+            A run[V];
+            A t;
+            t.init(p.identity);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                if (v < nv[u]) t.add(reg_put<T>(x[u][v]));
+                run[v] = t;
+            }
+            A incl = t;
+            for (int d = 1; d < 64; d <<= 1) {
+                A y = incl.shfl_up(d);
+                if (lane >= d) {
+                    y.merge(incl);
+                    incl = y;
+                }
+            }
+            A ex = incl.shfl_up(1);
+            if (lane == 0) ex.init(p.identity);
+            A pre = carry;
+            pre.merge(ex);
+            Reg o[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                A q = pre;
+                q.merge(run[v]);
+                const Reg val = q.value(p.rdt);
+                o[v] = odt == XG_BFLOAT16 ? cvt(val, p.rdt, XG_FLOAT32) : cvt(val, p.rdt, odt);
+            }
+            const uint32_t j = base + uint32_t(u * 64 * V) + uint32_t(lane * V);
+            char* ptr = out + int64_t(j) * oesz;
+            if (nv[u] == V) {
+                store_vec<V>(ptr, oesz, odt, o);
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v)
+                    if (v < nv[u]) store_raw(ptr + int64_t(v) * oesz, oesz, to_storage(o[v], odt));
+            }
+            carry.merge(incl.shfl_idx(63));
+        }
+    }
+}
+
 using SFn = void (*)(const ScanPlan);
 
 struct ScanKernels {
-    SFn rows32, rows64, cols32, cols64;
+    SFn rows32, rows64, cols32, cols64, rows_direct;
 };
 
 template <int AK, int OP> ScanKernels scan_kernels() {
     return ScanKernels{scan_rows_kernel<4, kMaxRegs, DynSpec, false, AK, OP>,
                        scan_rows_kernel<4, kMaxRegs, DynSpec, true, AK, OP>,
                        scan_cols_kernel<4, kMaxRegs, DynSpec, false, AK, OP>,
-                       scan_cols_kernel<4, kMaxRegs, DynSpec, true, AK, OP>};
+                       scan_cols_kernel<4, kMaxRegs, DynSpec, true, AK, OP>,
+                       scan_rows_direct_kernel<AK, OP>};
 }
 
 bool scan_kernels_float(int ak, int op, ScanKernels* k);
