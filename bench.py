@@ -426,6 +426,16 @@ def main():
     achieved = nbytes / (kms * 1e-3) / 1e9
 
     extra = {"kernel": kernel_name, "kernel_ms": round(kms, 5)}
+    # the same kernel with one and with four work units per thread
+    per_upt = {}
+    for upt in (1, 4):
+        A.lib().xg_debug_units_per_thread(upt)
+        try:
+            _, ukms = time_plans(torch, plans, max(10, args.steps // 2), 3)
+        finally:
+            A.lib().xg_debug_units_per_thread(0)
+        per_upt[str(upt)] = round(nbytes / (ukms * 1e-3) / 1e9, 1)
+    extra["GBps_by_units_per_thread"] = per_upt
     # the same workload on the generic interpreter (no compiled specialisation)
     A.lib().xg_debug_force_dynamic(1)
     try:

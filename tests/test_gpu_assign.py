@@ -98,6 +98,33 @@ def test_spec_and_dynamic_agree_bitwise():
     assert same_bits(c1.to_numpy(), c2.to_numpy())
 
 
+@pytest.mark.parametrize("shape", [(257, 1024), (3, 5), (1000, 1001), (64, 4096)])
+def test_units_per_thread_variants_agree(shape):
+    """One or four work units per thread (the large-launch form) give the same bits."""
+    a_h, b_h = make_ab(*shape, seed=4)
+    a, b = xt.from_numpy(a_h), xt.from_numpy(b_h)
+    outs = []
+    try:
+        for upt in (1, 4):
+            A.lib().xg_debug_units_per_thread(upt)
+            c = xt.empty(a_h.shape, "float64")
+            xt.noalias(c).assign(a * b + xt.exp(-a * a))
+            if shape[1] % 2 == 0:  # vector-aligned rows run the compiled program
+                assert xt.last_kernel() == "Spec_mul_add_exp_f64"
+            x = xt.from_numpy(np.random.default_rng(5).normal(0, 1, (3, 256, 56, 56)).astype(np.float32))
+            y = xt.zeros(x.shape, "float32")
+            params = [xt.from_numpy(np.full((1, 256, 1, 1), v, np.float32)) for v in (0.1, 1.5, 0.9, -0.2)]
+            vx = x[:, :, xt.range_(0, 56, 2), :]
+            xt.noalias(y[:, :, xt.range_(0, 56, 2), :]).assign(
+                xt.clip((vx - params[0]) / xt.sqrt(params[1] + 1e-5) * params[2] + params[3], -3, 3))
+            assert xt.last_kernel() == "Spec_affine_clip_f32"
+            sync()
+            outs.append((c.to_numpy(), y.to_numpy()))
+    finally:
+        A.lib().xg_debug_units_per_thread(0)
+    assert same_bits(outs[0][0], outs[1][0]) and same_bits(outs[0][1], outs[1][1])
+
+
 # ---------------------------------------------------------------- affine + clip through a strided view
 def affine_inputs(n=4, c=256, h=56, w=56, seed=1, dtype=np.float32):
     rng = np.random.default_rng(seed)

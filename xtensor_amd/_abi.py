@@ -178,6 +178,7 @@ EXPORTS = {
 DEBUG_EXPORTS = {
     "xg_last_kernel": (C.c_char_p, []),
     "xg_debug_force_dynamic": (None, [C.c_int]),
+    "xg_debug_units_per_thread": (None, [C.c_int]),
     "xg_debug_assign_kernel": (C.c_int, [_P(Expr), _P(Tensor), C.c_char_p, C.c_int32]),
 }
 

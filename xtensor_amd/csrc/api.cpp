@@ -374,6 +374,10 @@ void xg_plan_destroy(xg_plan_t plan) { delete plan; }
 // Testing aid: 1 routes every assignment through the dynamic interpreter.
 void xg_debug_force_dynamic(int on) { ew_set_force_dynamic(on); }
 
+// Measurement aid: 1 or 4 work units per thread in compiled elementwise
+// kernels, 0 = chosen by size.
+void xg_debug_units_per_thread(int upt) { ew_set_units_per_thread(upt); }
+
 // Name of the compiled specialisation an assignment would use, "" if none.
 xg_status xg_debug_assign_kernel(const xg_expr* e, const xg_tensor* out, char* name, int32_t cap) {
     return guard([&] {

@@ -108,6 +108,57 @@ __global__ __launch_bounds__(256) void ew_spec_kernel(const EwPlan plan) {
     for (uint32_t u = blockIdx.x * 256u + threadIdx.x; u < nunits; u += step) spec_unit<V, NR, S, RANK>(plan, u);
 }
 
+// UPT units per thread (u, u + 256, ...: every wave access stays one
+// contiguous run), all their loads issued before the first store, for more
+// bytes in flight per wave on large launches.
+template <int V, int NR, class S, int RANK, int UPT>
+__global__ __launch_bounds__(256) void ew_spec_multi_unit_kernel(const EwPlan plan) {
+    const uint32_t nunits = uint32_t(plan.units);
+    const uint32_t step = gridDim.x * 256u * UPT;
+    for (uint32_t b = blockIdx.x * 256u * UPT + threadIdx.x; b < nunits; b += step) {
+        Reg R[UPT][NR][V];
+        uint32_t idx[UPT][4];
+        uint32_t j0[UPT];
+        int nvalid[UPT];
+#pragma unroll
+        for (int k = 0; k < UPT; ++k) {
+            const uint32_t u = b + uint32_t(k) * 256u;
+            nvalid[k] = 0;
+            if (u < nunits) {
+                const uint32_t row = fastdiv(u, plan.div_chunks);
+                j0[k] = (u - row * plan.nchunks) * V;
+                idx[k][0] = idx[k][1] = idx[k][2] = idx[k][3] = 0;
+                uint32_t r = row;
+#pragma unroll
+                for (int d = RANK - 2; d >= 1; --d) {
+                    const uint32_t q = fastdiv(r, plan.div[d]);
+                    idx[k][d] = r - q * plan.shape32[d];
+                    r = q;
+                }
+                if constexpr (RANK >= 2) idx[k][0] = r;
+                nvalid[k] = min(V, int(plan.inner - j0[k]));
+                run_spec<S, 0, V, NR, RANK>(plan, idx[k], j0[k], nvalid[k], R[k]);
+            }
+        }
+        constexpr int odt = S::out_dtype;
+        constexpr int esz = dtype_size(odt);
+#pragma unroll
+        for (int k = 0; k < UPT; ++k) {
+            if (nvalid[k] <= 0) continue;
+            const int32_t off = coff<RANK, true>(plan.cst[kMaxLeaves], idx[k], j0[k]);
+            char* p = plan.out_ptr + int64_t(off) * esz;
+            if (S::out_mode == LM_CONTIG && nvalid[k] == V) {
+                store_vec<V>(p, esz, odt, R[k][S::result_reg]);
+            } else {
+                const int32_t os = plan.cst[kMaxLeaves][RANK - 1];
+#pragma unroll
+                for (int v = 0; v < V; ++v)
+                    if (v < nvalid[k]) store_raw(p + int64_t(v) * os * esz, esz, to_storage(R[k][S::result_reg][v], odt));
+            }
+        }
+    }
+}
+
 // Many expressions of one compiled program in one launch: the work units of
 // all plans are numbered consecutively (prefix[i] = first unit of plan i,
 // prefix[n] = total); a thread finds its plan by binary search.
@@ -144,6 +195,7 @@ struct SpecEntry {
     SpecKey key;
     EwFn by_rank[4];          // compiled kernels for normalised rank 1..4
     EwMultiFn multi_by_rank[4];  // fused many-expression kernels, rank 1..4
+    EwFn wide_by_rank[4];     // four units per thread, for large launches
 };
 
 template <class S> SpecKey key_of() {
@@ -170,7 +222,11 @@ std::vector<SpecEntry>& spec_table() {
                           {ew_spec_kernel<S::V, S::nregs, S, 1>, ew_spec_kernel<S::V, S::nregs, S, 2>,          \
                            ew_spec_kernel<S::V, S::nregs, S, 3>, ew_spec_kernel<S::V, S::nregs, S, 4>},         \
                           {ew_multi_kernel<S::V, S::nregs, S, 1>, ew_multi_kernel<S::V, S::nregs, S, 2>,        \
-                           ew_multi_kernel<S::V, S::nregs, S, 3>, ew_multi_kernel<S::V, S::nregs, S, 4>}});
+                           ew_multi_kernel<S::V, S::nregs, S, 3>, ew_multi_kernel<S::V, S::nregs, S, 4>},       \
+                          {ew_spec_multi_unit_kernel<S::V, S::nregs, S, 1, 4>,                                  \
+                           ew_spec_multi_unit_kernel<S::V, S::nregs, S, 2, 4>,                                  \
+                           ew_spec_multi_unit_kernel<S::V, S::nregs, S, 3, 4>,                                  \
+                           ew_spec_multi_unit_kernel<S::V, S::nregs, S, 4, 4>}});
         XG_SPEC_LIST(XG_REGISTER_SPEC)
 #undef XG_REGISTER_SPEC
     });
@@ -178,10 +234,12 @@ std::vector<SpecEntry>& spec_table() {
 }
 
 int g_force_dynamic = 0;
+int g_units_per_thread = 0;  // 0: choose by size; 1 or 4: forced (measurement aid)
 
 }  // namespace
 
 void ew_set_force_dynamic(int on) { g_force_dynamic = on; }
+void ew_set_units_per_thread(int upt) { g_units_per_thread = (upt == 1 || upt == 4) ? upt : 0; }
 
 const char* ew_find_spec(const EwPlan& p, int V) {
     if (g_force_dynamic || !p.compact) return nullptr;
@@ -250,7 +308,17 @@ hipError_t launch_ew(const EwPlan& p, int V, int cus, hipStream_t stream, const 
         for (auto& s : spec_table()) {
             if (s.V == V && spec_equal(s.key, k)) {
                 if (used) *used = s.name;
-                hipLaunchKernelGGL(s.by_rank[p.rank - 1], grid, block, 0, stream, p);
+                // One unit per thread unless forced: four units per thread
+                // measured slower on MI355X (lower occupancy: 2.74 vs 2.93 TB/s
+                // on the strided-view affine workload, 5.3 vs 5.6 TB/s on the
+                // contiguous int32 bit-mix).
+                const bool wide = g_units_per_thread == 4;
+                if (wide) {
+                    dim3 g4(ew_grid((p.units + 3) / 4, cus));
+                    hipLaunchKernelGGL(s.wide_by_rank[p.rank - 1], g4, block, 0, stream, p);
+                } else {
+                    hipLaunchKernelGGL(s.by_rank[p.rank - 1], grid, block, 0, stream, p);
+                }
                 return hipGetLastError();
             }
         }

@@ -16,6 +16,9 @@ const char* ew_find_spec(const EwPlan& p, int V);
 
 // Route every launch through the dynamic interpreter (testing aid).
 void ew_set_force_dynamic(int on);
+// Force one (1) or four (4) work units per thread in compiled kernels, or
+// choose by size (0) (measurement aid).
+void ew_set_units_per_thread(int upt);
 
 // One expression of a fused launch: the compact (rank <= 4, 32-bit offset)
 // part of an EwPlan that a compiled program needs at run time.

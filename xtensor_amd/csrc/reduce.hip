@@ -80,12 +80,17 @@ const RSpecEntry* find_rspec(const RedPlan& p, int V) {
 
 // Combine split partials: a thread per output when there are many outputs,
 // a block per output when few outputs carry many partials each.
-static hipError_t launch_combine(const RedPlan& q, hipStream_t stream) {
-    const bool wide = q.splits >= 64 && q.n_out < 4096;
-    KFn comb = combine_kernel(q.acc, q.op, wide);
+static hipError_t launch_combine(const RedPlan& p, hipStream_t stream) {
+    const bool wide = p.splits >= 1024 && p.n_out < 256;
+    KFn comb = combine_kernel(p.acc, p.op, wide);
     if (!comb) return hipErrorInvalidValue;
+    RedPlan q = p;
+    // lanes per output: about 8 partials per lane, at most a wave
+    uint32_t T = 1;
+    while (T < 64 && T * 8 < uint32_t(p.splits)) T <<= 1;
+    q.group = int(T);
     const uint64_t blocks = wide ? std::min<uint64_t>(q.n_out, 1u << 20)
-                                 : std::min<uint64_t>((q.n_out + 255) / 256, 1u << 20);
+                                 : std::min<uint64_t>((uint64_t(q.n_out) * T + 255) / 256, 1u << 20);
     hipLaunchKernelGGL(comb, dim3(uint32_t(std::max<uint64_t>(blocks, 1))), dim3(256), 0, stream, q);
     return hipGetLastError();
 }
@@ -122,7 +127,7 @@ hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** use
             const uint64_t tiles = (q.nchunks_f + 63) / 64;
             const uint64_t blocks = tiles * uint64_t(q.splits);
             if (blocks >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
-            hipLaunchKernelGGL(ks.cols_fast, dim3(uint32_t(blocks)), block, 0, stream, q);
+            hipLaunchKernelGGL(ks.cols_fast, dim3(uint32_t(blocks)), dim3(64 * kColsWaves), 0, stream, q);
         }
         hipError_t err = hipGetLastError();
         if (err != hipSuccess || q.splits == 1) return err;

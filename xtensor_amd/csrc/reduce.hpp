@@ -30,6 +30,10 @@ enum RedOp : int32_t { RO_ADD = 0, RO_MUL, RO_MAX, RO_MIN, RO_NUM };
 
 enum Finalize : int32_t { FZ_NONE = 0, FZ_MEAN = 1, FZ_SQRT = 2 };
 
+// waves per workgroup of the compiled cols kernel (they split the rows of
+// one column tile and combine in LDS)
+constexpr int kColsWaves = 8;
+
 struct RedPlan {
     int32_t layout;
     int32_t rank;       // normalised dims (see below)

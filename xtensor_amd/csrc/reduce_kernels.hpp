@@ -236,33 +236,58 @@ __global__ __launch_bounds__(256) void reduce_cols_kernel(const RedPlan p) {
     }
 }
 
+// combine within aligned groups of G (power of two <= 64) lanes of a wave,
+// lower lane's value first; every lane of the wave must call it
+template <class A> __device__ __forceinline__ A wave_group_combine(A a, int G) {
+    const int lane = threadIdx.x & 63;
+    for (int m = 1; m < G; m <<= 1) {
+        A o = a.shfl_xor(m);
+        if (lane & m) {
+            o.merge(a);
+            a = o;
+        } else {
+            a.merge(o);
+        }
+    }
+    return a;
+}
+
 // ---------------------------------------------------------------- split combine
-// a thread per output: partials merged in split order, loaded 8 at a time
+// p.group (T, a power of two <= 64) lanes per output: lane t merges partials
+// t, t + T, ... in order (loaded 8 at a time), then the T lanes combine in a
+// fixed butterfly, lower lane first
 template <int AK, int OP>
 __global__ __launch_bounds__(256) void reduce_combine_kernel(const RedPlan p) {
     using A = Acc<AK, OP>;
-    for (uint32_t o = blockIdx.x * 256 + threadIdx.x; o < p.n_out; o += gridDim.x * 256) {
-        const Store* part = reinterpret_cast<const Store*>(p.partials) + size_t(o) * p.splits;
+    const uint32_t T = uint32_t(p.group);
+    const uint32_t per_block = 256u / T;
+    const uint32_t t = threadIdx.x & (T - 1);
+    for (uint32_t ob = blockIdx.x * per_block; ob < p.n_out; ob += gridDim.x * per_block) {
+        const uint32_t o = ob + threadIdx.x / T;  // uniform over each group of T lanes
         A a;
-        a.load(part[0]);
-        int s = 1;
-        for (; s + 8 <= p.splits; s += 8) {
-            Store st[8];
+        a.init(p.identity);
+        if (o < p.n_out) {
+            const Store* part = reinterpret_cast<const Store*>(p.partials) + size_t(o) * p.splits;
+            uint32_t s = t;
+            for (; s + 7 * T < uint32_t(p.splits); s += 8 * T) {
+                Store st[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) st[k] = part[s + k];
+                for (int k = 0; k < 8; ++k) st[k] = part[s + uint32_t(k) * T];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
+                for (int k = 0; k < 8; ++k) {
+                    A b;
+                    b.load(st[k]);
+                    a.merge(b);
+                }
+            }
+            for (; s < uint32_t(p.splits); s += T) {
                 A b;
-                b.load(st[k]);
+                b.load(part[s]);
                 a.merge(b);
             }
         }
-        for (; s < p.splits; ++s) {
-            A b;
-            b.load(part[s]);
-            a.merge(b);
-        }
-        finish(p, a, o);
+        a = wave_group_combine(a, int(T));
+        if (o < p.n_out && t == 0) finish(p, a, o);
     }
 }
 
@@ -294,20 +319,10 @@ __global__ __launch_bounds__(256) void reduce_combine_wide_kernel(const RedPlan 
 // (no LDS, no barrier).  p.group / p.splits / p.per_split hold the fast
 // split (f_group / f_splits / f_per_split of the plan).
 
-template <int NR, int V> constexpr int fast_unroll() { return NR * V <= 8 ? 4 : (NR * V <= 16 ? 2 : 1); }
-
-template <class A> __device__ __forceinline__ A wave_group_combine(A a, int G) {
-    const int lane = threadIdx.x & 63;
-    for (int m = 1; m < G; m <<= 1) {
-        A o = a.shfl_xor(m);
-        if (lane & m) {  // lower lane's value first
-            o.merge(a);
-            a = o;
-        } else {
-            a.merge(o);
-        }
-    }
-    return a;
+// chunks (rows) in flight per lane: the rows kernel keeps more waves
+// resident with 4, the cols kernel gains from 8 on one-register programs
+template <int NR, int V, bool COLS> constexpr int fast_unroll() {
+    return (COLS && NR * V <= 4) ? 8 : (NR * V <= 8 ? 4 : (NR * V <= 16 ? 2 : 1));
 }
 
 // rows: dims [outputs | fast] (RANK 2) or [fast] (RANK 1, one output);
@@ -315,7 +330,7 @@ template <class A> __device__ __forceinline__ A wave_group_combine(A a, int G) {
 template <int V, int NR, class S, int AK, int OP, int RANK>
 __global__ __launch_bounds__(256) void reduce_rows_fast_kernel(const RedPlan p) {
     using A = Acc<AK, OP>;
-    constexpr int U = fast_unroll<NR, V>();
+    constexpr int U = fast_unroll<NR, V, false>();
     const int G = p.group;
     const int lane = threadIdx.x & 63;
     const uint32_t per_wave = 64u / uint32_t(G);
@@ -355,12 +370,14 @@ __global__ __launch_bounds__(256) void reduce_rows_fast_kernel(const RedPlan p) 
 }
 
 // cols: dims [rows | fast outputs]; a block = one tile of 64 * V outputs and
-// one split of the rows, its four waves interleaving rows, combined in LDS
+// one split of the rows, its kColsWaves waves interleaving rows, combined in
+// LDS in wave order
 template <int V, int NR, class S, int AK, int OP>
-__global__ __launch_bounds__(256) void reduce_cols_fast_kernel(const RedPlan p) {
+__global__ __launch_bounds__(64 * kColsWaves) void reduce_cols_fast_kernel(const RedPlan p) {
     using A = Acc<AK, OP>;
-    constexpr int U = fast_unroll<NR, V>();
-    __shared__ Store lds[4][64][V];
+    constexpr int U = fast_unroll<NR, V, true>();
+    constexpr int W = kColsWaves;
+    __shared__ Store lds[W][64][V];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const uint32_t tile = fastdiv(blockIdx.x, p.div_splits);
@@ -375,12 +392,12 @@ __global__ __launch_bounds__(256) void reduce_cols_fast_kernel(const RedPlan p) 
     if (active) {
         const uint32_t r0 = sp * p.per_split;
         const uint32_t r1 = min(p.red_count, r0 + p.per_split);
-        for (uint32_t r = r0 + uint32_t(wave); r < r1; r += 4 * U) {
+        for (uint32_t r = r0 + uint32_t(wave); r < r1; r += W * U) {
             Reg R[U][NR][V];
             bool ok[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint32_t rr = r + uint32_t(4 * u);
+                const uint32_t rr = r + uint32_t(W * u);
                 ok[u] = rr < r1;
                 const uint32_t idx[4] = {rr, 0u, 0u, 0u};
                 if (ok[u]) run_spec<S, 0, V, NR, 2>(p, idx, j0, nvalid, R[u]);
@@ -402,7 +419,7 @@ __global__ __launch_bounds__(256) void reduce_cols_fast_kernel(const RedPlan p) 
             if (v < nvalid) {
                 A a;
                 a.load(lds[0][lane][v]);
-                for (int w = 1; w < 4; ++w) {
+                for (int w = 1; w < W; ++w) {
                     A b;
                     b.load(lds[w][lane][v]);
                     a.merge(b);

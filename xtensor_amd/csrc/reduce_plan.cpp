@@ -500,11 +500,11 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
         } else {
             const uint64_t rows = p.red_count;
             const uint64_t tiles = (p.nchunks_f + 63) / 64;
-            const uint64_t target_blocks = target_waves / 4;
+            const uint64_t target_blocks = target_waves / kColsWaves;
             uint64_t s = (target_blocks + tiles - 1) / tiles;
-            s = std::min<uint64_t>(s, std::max<uint64_t>(1, rows / (4 * kU * 2)));
-            // partial results stay within a few percent of the input bytes
-            s = std::min<uint64_t>(s, std::max<uint64_t>(1, uint64_t(best) / (uint64_t(n_out) * 16 * 32)));
+            s = std::min<uint64_t>(s, std::max<uint64_t>(1, rows / (kColsWaves * kU * 2)));
+            // partial results stay within about 6% of the input bytes
+            s = std::min<uint64_t>(s, std::max<uint64_t>(1, uint64_t(best) / (uint64_t(n_out) * 16 * 16)));
             p.f_group = 256;
             p.f_splits = int(std::min<uint64_t>(s, 65535));
             p.f_per_split = uint32_t((rows + p.f_splits - 1) / p.f_splits);
