@@ -180,66 +180,77 @@ def uneven_batch(np, torch, xt, steps=20, n_inst=4096):
     }
 
 
-def reductions(np, torch, xt, steps=10):
+def reductions(np, torch, xt, steps=12, copies=3):
     """Axis reductions and the scan on X float32 [8192, 4096] ~ N(1, 1) (seed 2),
     its bfloat16 copy, its transpose view and an int32 copy U[-1000, 1000];
     plus the fused broadcast expression of a 2048 x 1024 float64 pair.  Each
     result: kernel time, GB/s of algorithmic bytes (input read once + output
-    written) and the relative error against float64 (int: exact)."""
+    written) and the relative error against float64 (int: exact).  The large
+    inputs rotate over `copies` device copies (with their own outputs), so
+    consecutive launches never find their operand in the 256 MB MALL."""
     rng = np.random.default_rng(2)
     R, Cc = 8192, 4096
     X = rng.normal(1.0, 1.0, (R, Cc)).astype(np.float32)
     Xi = rng.integers(-1000, 1001, (R, Cc), dtype=np.int32)
-    dX = xt.from_numpy(X)
-    dXb = xt.from_numpy(X, "bfloat16")
-    dXi = xt.from_numpy(Xi)
+    dX = [xt.from_numpy(X) for _ in range(copies)]
+    dXb = [xt.from_numpy(X, "bfloat16") for _ in range(copies + 1)]  # 4 x 67 MB > MALL
+    dXi = [xt.from_numpy(Xi) for _ in range(copies)]
     X64 = X.astype(np.float64)
     Xb64 = torch.from_numpy(X).to(torch.bfloat16).to(torch.float64).numpy()
     out = {}
 
-    def run(name, e, ref, in_bytes, exact=False, dtype=None):
-        if isinstance(e, xt.Reducer):
-            dt, shp = xt.reduce_info(e)
-            o = xt.empty(shp, dtype or xt.dtype_name(dt))
-        else:
-            o = xt.empty(xt.expr_info(e)[1], dtype or xt.dtype_name(xt.expr_info(e)[0]))
-        p = xt.plan(o, e)
-        _, kms = time_plans(torch, [p], steps, 2)
-        got = o.to_numpy().astype(np.float64)
+    def run(name, make, srcs, ref, in_bytes, exact=False, dtype=None):
+        plans, outs = [], []
+        for src in srcs:
+            e = make(src)
+            if isinstance(e, xt.Reducer):
+                dt, shp = xt.reduce_info(e)
+                o = xt.empty(shp, dtype or xt.dtype_name(dt))
+            else:
+                o = xt.empty(xt.expr_info(e)[1], dtype or xt.dtype_name(xt.expr_info(e)[0]))
+            plans.append(xt.plan(o, e))
+            outs.append(o)
+        _, kms = time_plans(torch, plans, steps, len(plans))
         ref = np.asarray(ref, np.float64)
-        if exact:
-            err = 0.0 if np.array_equal(got, ref) else float("inf")
-        else:
-            err = float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)))
+        err = 0.0
+        for o in outs:
+            got = o.to_numpy().astype(np.float64)
+            if exact:
+                err = max(err, 0.0 if np.array_equal(got, ref) else float("inf"))
+            else:
+                err = max(err, float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300))))
+        o = outs[0]
         nbytes = in_bytes + o.size * o.itemsize
         out[name] = {"kernel_ms": round(kms, 4), "GBps": round(nbytes / (kms * 1e-3) / 1e9, 1),
                      "max_rel_err_vs_fp64": err, "kernel": xt.last_kernel()}
-        del o, p
+        del outs, plans
 
     nb = R * Cc * 4
-    run("sum_axis0_f32", xt.sum(dX, [0]), X64.sum(0), nb)
-    run("sum_axis1_f32", xt.sum(dX, [1]), X64.sum(1), nb)
-    run("mean_axis0_f32", xt.mean(dX, [0]), X64.mean(0), nb)
-    run("mean_axis1_f32", xt.mean(dX, [1]), X64.mean(1), nb)
-    run("amax_all_f32", xt.amax(dX), X64.max(), nb, exact=True)
-    run("norm_l2_all_f32", xt.norm_l2(dX), np.sqrt((X64 * X64).sum()), nb)
-    run("sum_axis0_transposed_f32", xt.sum(dX.T, [0]), X64.sum(1), nb)
-    run("sum_axis1_transposed_f32", xt.sum(dX.T, [1]), X64.sum(0), nb)
-    run("sum_axis0_bf16", xt.sum(dXb, [0]), Xb64.sum(0), nb // 2)
-    run("sum_axis1_bf16", xt.sum(dXb, [1]), Xb64.sum(1), nb // 2)
-    run("sum_int64_axis0_i32", xt.sum(dXi, [0], dtype="int64"), Xi.astype(np.int64).sum(0), nb, exact=True)
-    run("sum_int64_axis1_i32", xt.sum(dXi, [1], dtype="int64"), Xi.astype(np.int64).sum(1), nb, exact=True)
-    # cumsum along axis 1 (a scan plan)
-    cs = xt.empty((R, Cc), "float32")
-    pcs = xt.plan(cs, xt.Scan("cumsum", dX, 1))
-    _, kms = time_plans(torch, [pcs], steps, 2)
+    run("sum_axis0_f32", lambda d: xt.sum(d, [0]), dX, X64.sum(0), nb)
+    run("sum_axis1_f32", lambda d: xt.sum(d, [1]), dX, X64.sum(1), nb)
+    run("mean_axis0_f32", lambda d: xt.mean(d, [0]), dX, X64.mean(0), nb)
+    run("mean_axis1_f32", lambda d: xt.mean(d, [1]), dX, X64.mean(1), nb)
+    run("amax_all_f32", lambda d: xt.amax(d), dX, X64.max(), nb, exact=True)
+    run("norm_l2_all_f32", lambda d: xt.norm_l2(d), dX, np.sqrt((X64 * X64).sum()), nb)
+    run("sum_axis0_transposed_f32", lambda d: xt.sum(d.T, [0]), dX, X64.sum(1), nb)
+    run("sum_axis1_transposed_f32", lambda d: xt.sum(d.T, [1]), dX, X64.sum(0), nb)
+    run("sum_axis0_bf16", lambda d: xt.sum(d, [0]), dXb, Xb64.sum(0), nb // 2)
+    run("sum_axis1_bf16", lambda d: xt.sum(d, [1]), dXb, Xb64.sum(1), nb // 2)
+    run("sum_int64_axis0_i32", lambda d: xt.sum(d, [0], dtype="int64"), dXi, Xi.astype(np.int64).sum(0), nb,
+        exact=True)
+    run("sum_int64_axis1_i32", lambda d: xt.sum(d, [1], dtype="int64"), dXi, Xi.astype(np.int64).sum(1), nb,
+        exact=True)
+    # cumsum along axis 1 (scan plans, one output per input copy)
+    css = [xt.empty((R, Cc), "float32") for _ in dX]
+    pcs = [xt.plan(cs, xt.Scan("cumsum", d, 1)) for cs, d in zip(css, dX)]
+    _, kms = time_plans(torch, pcs, steps, len(pcs))
     ref = np.cumsum(X64, axis=1)
-    got = cs.to_numpy().astype(np.float64)
+    got = css[0].to_numpy().astype(np.float64)
     out["cumsum_axis1_f32"] = {"kernel_ms": round(kms, 4), "GBps": round(2 * nb / (kms * 1e-3) / 1e9, 1),
                                "max_abs_err_over_rowsum_abs": float(np.max(np.abs(got - ref) /
                                                                     np.abs(X64).sum(1, keepdims=True))),
                                "kernel": xt.last_kernel()}
-    del cs, pcs
+    del css, pcs, dX, dXb, dXi
     # the broadcast expression of 2048 x 1024 float64 and its reductions
     r1 = np.random.default_rng(0)
     a = r1.uniform(-1, 1, (2048, 1024))
@@ -253,8 +264,10 @@ def reductions(np, torch, xt, steps=10):
         "kernel_ms": round(kms, 4), "GBps": round((2 * 2048 * 1024 + 1024) * 8 / (kms * 1e-3) / 1e9, 1),
         "max_rel_err_vs_host_libm": float(np.max(np.abs(c.to_numpy() - c_ref) / np.maximum(np.abs(c_ref), 1e-300))),
         "kernel": xt.last_kernel()}
-    run("sum_axis1_of_c_f64", xt.sum(c, [1]), c_ref.sum(1), 2048 * 1024 * 8)
-    run("amax_axis0_of_c_f64", xt.amax(c, [0]), c.to_numpy().max(0), 2048 * 1024 * 8, exact=True)
+    # the 16 MB result c stays cache-resident between these small launches,
+    # as it would in the pipeline c -> sum(c) -> amax(c)
+    run("sum_axis1_of_c_f64", lambda d: xt.sum(d, [1]), [c], c_ref.sum(1), 2048 * 1024 * 8)
+    run("amax_axis0_of_c_f64", lambda d: xt.amax(d, [0]), [c], c.to_numpy().max(0), 2048 * 1024 * 8, exact=True)
     return out
 
 
@@ -444,6 +457,19 @@ def main():
         extra["generic_interpreter_GBps"] = round(nbytes / (dkms * 1e-3) / 1e9, 1)
     finally:
         A.lib().xg_debug_force_dynamic(0)
+    # ceilings of the access pattern on this GPU: a plain copy through the
+    # same views, and a contiguous copy of the same number of bytes (both
+    # rotating over the three sets, through the compiled copy program)
+    touched = N * C * (H // 2) * W
+    cplans = [xt.plan(xt.view(y, *view), xt.view(x, *view)) for x, y in sets]
+    _, ckms = time_plans(torch, cplans, max(10, args.steps // 2), 3)
+    extra["view_copy_GBps"] = round(8 * touched / (ckms * 1e-3) / 1e9, 1)
+    extra["view_copy_kernel"] = xt.last_kernel()
+    fplans = [xt.plan(xt.DeviceArray(y.storage, "float32", (touched,), None, 0),
+                      xt.DeviceArray(x.storage, "float32", (touched,), None, 0)) for x, y in sets]
+    _, fkms = time_plans(torch, fplans, max(10, args.steps // 2), 3)
+    extra["contiguous_copy_same_bytes_GBps"] = round(8 * touched / (fkms * 1e-3) / 1e9, 1)
+    del cplans, fplans
     # int32 variant: y = (x*3 + (x>>2)) & 0xFFFF over the full array
     xi = xt.from_numpy(np.random.default_rng(1).integers(-2**20, 2**20, (N, C, H, W), dtype=np.int32))
     yi = xt.empty((N, C, H, W), "int32")

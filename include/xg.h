@@ -202,12 +202,18 @@ typedef enum xg_reduce_op {
     XG_RED_PROD = 1,      /* xt::prod<T>  (detail::multiplies, init 1)      */
     XG_RED_AMAX = 2,      /* xt::amax<T>  (maximum, init lowest())          */
     XG_RED_AMIN = 3,      /* xt::amin<T>  (minimum, init max())             */
-    XG_RED_MEAN = 4,      /* xt::mean<T>: sum<T> / double(n)                */
+    XG_RED_MEAN = 4,      /* xt::mean<T>: sum<T> / T(n), T = double if unset */
     XG_RED_NORM_SQ = 5,   /* xt::norm_sq: sum of v*v in big_promote type    */
     XG_RED_NORM_L2 = 6,   /* xt::norm_l2 = sqrt(norm_sq)                    */
     XG_RED_NORM_L1 = 7,   /* xt::norm_l1: sum of abs(v)                     */
     XG_RED_NORM_LINF = 8, /* xt::norm_linf: max of abs(v)                   */
-    XG_RED_NORM_L0 = 9    /* xt::norm_l0: count of v != 0                   */
+    XG_RED_NORM_L0 = 9,   /* xt::norm_l0: count of v != 0                   */
+    XG_RED_VARIANCE = 10, /* xt::variance<T>(e, axes, ddof): mean<T> of       */
+                          /* square(e - mean<T>(e, axes)), core/xmath.hpp:2031-2113 */
+    XG_RED_STDDEV = 11,   /* xt::stddev<T> = sqrt(variance<T>), core/xmath.hpp:2046, 2143 */
+    XG_RED_MINMAX = 12    /* xt::minmax(e): {min, max} over all elements into */
+                          /* a [2] output (std::min / std::max folds),        */
+                          /* core/xmath.hpp:2195-2230                          */
 } xg_reduce_op;
 
 typedef struct xg_reduce_desc {
@@ -219,7 +225,9 @@ typedef struct xg_reduce_desc {
     int32_t has_initial;  /* xt::initial(value) option                       */
     xg_scalar initial;
     int32_t flags;        /* reserved, 0                                      */
-    int32_t pad_;
+    int32_t ddof;         /* variance / stddev: delta degrees of freedom; the */
+                          /* divisor is (size - ddof) / outputs in size_t     */
+                          /* arithmetic, like detail::mean; 0 for other ops    */
 } xg_reduce_desc;
 
 /* ---- scans (accumulators) ------------------------------------------------ */

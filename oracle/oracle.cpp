@@ -25,9 +25,13 @@
 //     sequentially across outer reduced axes, `initial` merged last
 //     (reducers/xreducer.hpp:1778-1828); value type
 //     decltype(reduce(init_type, value_type)); sum/prod/amax/amin inits from
-//     core/xmath.hpp:367-423, 777-800, 1803-1823; mean = sum<T> / double(n)
-//     with n = e.size()/s.size() (core/xmath.hpp:1825-1874); norms from
-//     reducers/xnorm.hpp:225-290, 397-426, 493-526.
+//     core/xmath.hpp:367-423, 777-800, 1803-1823; mean = sum<T> / VT(n)
+//     with VT = T (double when T is not given) and n = (e.size() - ddof) /
+//     s.size() in size_t arithmetic (core/xmath.hpp:1825-1874); variance /
+//     stddev evaluate the inner mean into a temporary first, then the mean
+//     of square(e - m) with ddof (core/xmath.hpp:2031-2113); minmax folds
+//     std::min / std::max from {max(), lowest()} (core/xmath.hpp:2195-2230);
+//     norms from reducers/xnorm.hpp:225-290, 397-426, 493-526.
 //   * scans: copy then the in-place sequential recurrence
 //     res[k] = f(res[k-1], res[k]) along the axis
 //     (reducers/xaccumulator.hpp:215-297, 343-367), result type
@@ -731,7 +735,8 @@ int reduce_result_dt(int op, int acc, int edt) {
             case XG_RED_AMIN: return dt_of<decltype(xselect(IT() < E(), IT(), E()))>();
             case XG_RED_MEAN: {
                 using S = decltype(IT() + E());
-                return dt_of<decltype(S() / double())>();
+                if (acc < 0) return dt_of<decltype(S() / double())>();
+                return dt_of<decltype(S() / IT())>();  // mean_division's value_type = T
             }
             default: break;
             }
@@ -741,6 +746,22 @@ int reduce_result_dt(int op, int acc, int edt) {
             if (acc >= 0) return visit(acc, with_init);
             return with_init(te);
         }
+        if (op == XG_RED_VARIANCE || op == XG_RED_STDDEV) {
+            // E - M, M = mean<T>(e); its square; mean<T> of that; stddev: sqrt
+            int mdt = reduce_result_dt(XG_RED_MEAN, acc, edt);
+            int qdt = visit(mdt, [&](auto tm) -> int {
+                using M = typename decltype(tm)::type;
+                using D = decltype(E() - M());
+                return dt_of<decltype(D() * D())>();
+            });
+            int vdt = reduce_result_dt(XG_RED_MEAN, acc, qdt);
+            if (op == XG_RED_VARIANCE) return vdt;
+            return visit(vdt, [](auto tv) -> int {
+                using VV = typename decltype(tv)::type;
+                return dt_of<decltype(std::sqrt(VV()))>();
+            });
+        }
+        if (op == XG_RED_MINMAX) return edt;
         using BP = typename big_promote<E>::type;
         switch (op) {
         case XG_RED_NORM_SQ: return dt_of<decltype(BP() + E() * E())>();
@@ -753,7 +774,22 @@ int reduce_result_dt(int op, int acc, int edt) {
     });
 }
 
-void do_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, bool high_precision) {
+void do_moment(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, bool high_precision);
+void do_minmax(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out);
+
+// `ddof` and `sqrt_after` serve variance / stddev: the mean's divisor is
+// (size - ddof) / outputs and the result's square root is stored.
+void do_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, bool high_precision,
+               int64_t ddof = 0, bool sqrt_after = false) {
+    if (r->op == XG_RED_MINMAX) {
+        do_minmax(e, r, out);
+        return;
+    }
+    if (r->op == XG_RED_VARIANCE || r->op == XG_RED_STDDEV) {
+        do_moment(e, r, out, high_precision);
+        return;
+    }
+    if (r->ddof != 0) fail(XG_INVALID_ARGUMENT, "ddof applies to variance and stddev only");
     Tree t = analyse(e);
     const Shape& in = t.shape;
     std::vector<int> ax = normalise_axes(r, int(in.size()));
@@ -808,11 +844,11 @@ void do_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, 
             default: return a + b;
             }
         };
-        int64_t n_red = 1;
-        for (int a : ax) n_red *= in[size_t(a)];
+        const int64_t n_out_hp = numel(rs);
+        const int64_t div_hp = n_out_hp != 0 ? (numel(in) - ddof) / n_out_hp : 0;
         reduce_loop<LD>(ev, in, ax, keep, init, r->has_initial != 0, ini, fn, mg, [&](int64_t o, LD v) {
-            if (op == XG_RED_MEAN) v = v / LD(n_red);
-            if (op == XG_RED_NORM_L2) v = std::sqrt(v);
+            if (op == XG_RED_MEAN) v = v / LD(div_hp);
+            if (op == XG_RED_NORM_L2 || sqrt_after) v = std::sqrt(v);
             store(*out, emit_offset(o), make<double>(double(v)));
         });
         return;
@@ -871,15 +907,101 @@ void do_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, 
     };
     reduce_loop<Val>(ev, in, ax, keep, init, r->has_initial != 0, ini, fn, mg, [&](int64_t o, Val v) {
         if (op == XG_RED_MEAN) {
-            // mean_division: div = double(e.size() / s.size()), 0 when s is empty
-            int64_t esz = numel(in);
-            double div = n_out != 0 ? static_cast<double>(esz / n_out) : 0.0;
-            v = apply2<f_div>(v, make<double>(div));
+            // mean_division: div = value_type((e.size() - ddof) / s.size()) with
+            // value_type = T or double, 0 when s is empty
+            uint64_t q = n_out != 0 ? uint64_t(numel(in) - ddof) / uint64_t(n_out) : 0;
+            Val div = r->acc_dtype >= 0 ? convert(make<uint64_t>(q), r->acc_dtype) : make<double>(double(q));
+            v = apply2<f_div>(v, div);
         } else if (op == XG_RED_NORM_L2) {
             v = apply1<f_sqrt>(v);
         }
+        if (sqrt_after) v = apply1<f_sqrt>(v);
         store(*out, emit_offset(o), v);
     });
+}
+
+// xt::variance / xt::stddev restated (core/xmath.hpp:2031-2113): the inner
+// mean<T>(e, axes) is evaluated into a temporary with the reduced dims kept,
+// then mean<T>(square(e - m), axes, ddof) -- square(x) is x * x -- and for
+// stddev its square root.
+void do_moment(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, bool high_precision) {
+    Tree t = analyse(e);
+    int edt = t.node_dt[size_t(e->n_nodes - 1)];
+    std::vector<int> ax = normalise_axes(r, int(t.shape.size()));
+    if (r->ddof < 0 || r->ddof > numel(t.shape)) fail(XG_INVALID_ARGUMENT, "ddof out of range");
+    xg_reduce_desc r1 = *r;
+    r1.op = XG_RED_MEAN;
+    r1.keep_dims = 1;
+    r1.has_initial = 0;
+    r1.ddof = 0;
+    Shape ms = reduce_shape(t.shape, ax, true);
+    int mdt = high_precision ? XG_FLOAT64 : reduce_result_dt(XG_RED_MEAN, r->acc_dtype, edt);
+    std::vector<uint64_t> mbuf(size_t(std::max<int64_t>(numel(ms), 1)));
+    xg_tensor m{};
+    m.data = mbuf.data();
+    m.dtype = mdt;
+    m.ndim = int32_t(ms.size());
+    int64_t st = 1;
+    for (int k = m.ndim - 1; k >= 0; --k) {
+        m.shape[k] = ms[size_t(k)];
+        m.strides[k] = st;
+        st *= ms[size_t(k)];
+    }
+    do_reduce(e, &r1, &m, high_precision);
+    std::vector<xg_node> nodes(e->nodes, e->nodes + e->n_nodes);
+    std::vector<xg_tensor> leaves(e->leaves, e->leaves + e->n_leaves);
+    leaves.push_back(m);
+    xg_node ln{};
+    ln.op = XG_OP_LEAF;
+    ln.index = e->n_leaves;
+    nodes.push_back(ln);
+    xg_node sub{};
+    sub.op = XG_OP_MINUS;
+    sub.nargs = 2;
+    sub.args[0] = e->n_nodes - 1;
+    sub.args[1] = e->n_nodes;
+    nodes.push_back(sub);
+    xg_node sq{};
+    sq.op = XG_OP_MULTIPLIES;
+    sq.nargs = 2;
+    sq.args[0] = e->n_nodes + 1;
+    sq.args[1] = e->n_nodes + 1;
+    nodes.push_back(sq);
+    xg_expr e2 = *e;
+    e2.n_nodes = int32_t(nodes.size());
+    e2.n_leaves = int32_t(leaves.size());
+    e2.nodes = nodes.data();
+    e2.leaves = leaves.data();
+    xg_reduce_desc r2 = *r;
+    r2.op = XG_RED_MEAN;
+    r2.ddof = 0;
+    do_reduce(&e2, &r2, out, high_precision, r->ddof, r->op == XG_RED_STDDEV);
+}
+
+// xt::minmax restated (core/xmath.hpp:2195-2230): r = {max(), lowest()} of
+// the value type; for every element v (row-major), r[0] = std::min(r[0], v),
+// r[1] = std::max(r[1], v).
+void do_minmax(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out) {
+    if (r->n_axes >= 0 || r->keep_dims || r->has_initial)
+        fail(XG_INVALID_ARGUMENT, "minmax reduces over all axes and takes no options");
+    Tree t = analyse(e);
+    int edt = t.node_dt[size_t(e->n_nodes - 1)];
+    if (out->ndim != 1 || out->shape[0] != 2) fail(XG_BROADCAST_ERROR, "minmax output must have shape (2)");
+    Val lo = visit(edt, [](auto tv) { using T = typename decltype(tv)::type; return make<T>(std::numeric_limits<T>::max()); });
+    Val hi = visit(edt, [](auto tv) { using T = typename decltype(tv)::type; return make<T>(std::numeric_limits<T>::lowest()); });
+    const Shape& in = t.shape;
+    if (numel(in) > 0) {
+        Evaluator ev(e, in);
+        int nd = int(in.size());
+        std::vector<int64_t> idx(size_t(nd), 0);
+        do {
+            Val v = convert(ev.at(idx.data(), nd), edt);
+            if (truth(apply2<f_lt>(v, lo))) lo = v;  // std::min(r, v) = v < r ? v : r
+            if (truth(apply2<f_lt>(hi, v))) hi = v;  // std::max(r, v) = r < v ? v : r
+        } while (nd > 0 && next_index(idx, in));
+    }
+    store(*out, out->offset, lo);
+    store(*out, out->offset + out->strides[0], hi);
 }
 
 // ---------------------------------------------------------------- scans
@@ -996,6 +1118,12 @@ int xo_assign(const xg_expr* e, const xg_tensor* out) {
 int xo_reduce_info(const xg_expr* e, const xg_reduce_desc* r, int32_t* dtype, int32_t* ndim, int64_t* shape) {
     return guard([&] {
         Tree t = analyse(e);
+        if (r->op == XG_RED_MINMAX) {
+            *dtype = t.node_dt[size_t(e->n_nodes - 1)];
+            *ndim = 1;
+            shape[0] = 2;
+            return;
+        }
         std::vector<int> ax = normalise_axes(r, int(t.shape.size()));
         Shape rs = reduce_shape(t.shape, ax, r->keep_dims != 0);
         int rdt = reduce_result_dt(r->op, r->acc_dtype, t.node_dt[size_t(e->n_nodes - 1)]);

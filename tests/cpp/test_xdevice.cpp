@@ -4,6 +4,7 @@
 //                      error mapping (no device memory is touched)
 //   test_xdevice gpu : evaluation on the device, checked against host loops
 // Prints "OK" and exits 0 on success.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -54,6 +55,15 @@ static void cpu_tests() {
     EXPECT(rt3 == XG_INT64);
     auto [rt4, rs4] = xt::reduce_info(xt::amax(fa, {0, 2}, xt::keep_dims));
     EXPECT(rt4 == XG_FLOAT32 && rs4 == xt::shape_type({1, 4, 1}));
+    // variance / stddev / minmax (core/xmath.hpp:2031-2230)
+    auto [rt5, rs5] = xt::reduce_info(xt::variance(fa, {0, 2}));
+    EXPECT(rt5 == XG_FLOAT64 && rs5 == xt::shape_type({4}));
+    auto [rt6, rs6] = xt::reduce_info(xt::stddev<float>(fa, {1}));
+    EXPECT(rt6 == XG_FLOAT32 && rs6 == xt::shape_type({2, 3}));
+    auto [rt7, rs7] = xt::reduce_info(xt::minmax(ia));
+    EXPECT(rt7 == XG_INT32 && rs7 == xt::shape_type({2}));
+    auto [rt8, rs8] = xt::reduce_info(xt::mean<int64_t>(ia, {0}));
+    EXPECT(rt8 == XG_INT64 && rs8.empty());
 
     // views: strides and offsets of views/xview.hpp
     auto y = xt::xdevice_array<float>::adapt(reinterpret_cast<float*>(fake), {8, 6, 10});
@@ -161,6 +171,18 @@ static void gpu_tests() {
     }
     auto st = xt::eval<int64_t>(xt::sum<int64_t>(ia, {0}));
     EXPECT(st.to_host()[0] == tot);
+
+    // variance with ddof and minmax on the device; small integers keep the
+    // sums exact, so the values are exact too
+    std::vector<double> hv = {1, 2, 3, 4, 6, 8};
+    auto dv = xt::xdevice_array<double>::from_host(hv, {2, 3});
+    auto var1 = xt::eval<double>(xt::variance(dv, {1}, 1)).to_host();  // rows {1,2,3}, {4,6,8}: 1, 4
+    EXPECT(var1.size() == 2 && var1[0] == 1.0 && var1[1] == 4.0);
+    auto sd = xt::eval<double>(xt::stddev(dv, {1})).to_host();
+    EXPECT(std::fabs(sd[0] - std::sqrt(2.0 / 3.0)) < 1e-15 && std::fabs(sd[1] - std::sqrt(8.0 / 3.0)) < 1e-15);
+    auto mm = xt::eval<int32_t>(xt::minmax(ia)).to_host();
+    EXPECT(mm.size() == 2 && mm[0] == *std::min_element(hi.begin(), hi.end()) &&
+           mm[1] == *std::max_element(hi.begin(), hi.end()));
 
     // cumsum along an axis (reducers/xaccumulator.hpp:282-294): exact on small integers
     std::vector<double> hsmall(6 * 5);

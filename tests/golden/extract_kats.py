@@ -9,7 +9,8 @@ For every listed test file, each array initialised with a braced list —
 `xarray<T> name = {...};`, `xtensor<T, N> name = {...};`, `xarray<T> name{...};`
 — becomes one record keyed "<file>:<TEST name>:<variable>" holding the C++
 element type, the shape implied by the nesting, the values (row-major) and
-the line number.  Only numbers are kept; the surrounding C++ is not copied.
+the line number; numpy-generated scalars `double py_name = v;` become records
+of shape [].  Only numbers are kept; the surrounding C++ is not copied.
 """
 
 from __future__ import annotations
@@ -28,6 +29,7 @@ FILES = [
     "test_xmath.cpp",
     "test_xbroadcast.cpp",
     "test_strided_assign.cpp",
+    "test_extended_xmath_reducers.cpp",
 ]
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kat_arrays.json")
 
@@ -35,6 +37,8 @@ DECL = re.compile(
     r"(?:xt::)?(xarray|xtensor)\s*<\s*([A-Za-z_:0-9 ]+?)\s*(?:,\s*\d+\s*)?>\s+([A-Za-z_][A-Za-z_0-9]*)\s*(=\s*)?\{")
 TEST = re.compile(r"TEST\s*\(\s*([A-Za-z_0-9]+)\s*,\s*([A-Za-z_0-9]+)\s*\)")
 NUM = re.compile(r"[-+]?(?:\d+\.\d*|\.\d+|\d+)(?:[eE][-+]?\d+)?")
+# numpy-generated scalar results: `double py_name = 0.123;`
+SCALAR = re.compile(r"\bdouble\s+(py_[A-Za-z_0-9]*)\s*=\s*([-+]?(?:\d+\.\d*|\.\d+|\d+)(?:[eE][-+]?\d+)?)\s*;")
 
 
 def braced(text: str, start: int) -> tuple[str, int]:
@@ -131,6 +135,16 @@ def main() -> int:
                 "shape": shape_of(val),
                 "values": flatten(val),
                 "line": line,
+            }
+        for m in SCALAR.finditer(text_n):
+            tname = ""
+            for pos, name in tests:
+                if pos < m.start():
+                    tname = name
+            line = text_n.count("\n", 0, m.start()) + 1
+            records[f"{fname}:{tname}:{m.group(1)}"] = {
+                "container": "scalar", "ctype": "double", "shape": [],
+                "values": [float(m.group(2))], "line": line,
             }
     with open(OUT, "w") as f:
         json.dump(records, f, indent=0, sort_keys=True)

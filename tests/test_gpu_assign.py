@@ -125,6 +125,38 @@ def test_units_per_thread_variants_agree(shape):
     assert same_bits(outs[0][0], outs[1][0]) and same_bits(outs[0][1], outs[1][1])
 
 
+def test_full_size_workloads_run_compiled_programs():
+    """At full size (operands above the non-temporal threshold, so their loads
+    and stores take the streaming form) the strided-view affine workload and
+    the int32 bit-mix still match their compiled programs, and the streaming
+    and cached forms give the same bits."""
+    rng = np.random.default_rng(8)
+    x = xt.from_numpy(rng.normal(0, 1, (64, 256, 56, 56)).astype(np.float32))
+    params = [xt.from_numpy(np.full((1, 256, 1, 1), v, np.float32)) for v in (0.1, 1.5, 0.9, -0.2)]
+    xi = xt.from_numpy(rng.integers(-2**20, 2**20, (64, 256, 56, 56), dtype=np.int32))
+    outs = []
+    try:
+        for nt in (1, 0):
+            A.lib().xg_debug_nontemporal(nt, nt)
+            y = xt.zeros(x.shape, "float32")
+            vx = x[:, :, xt.range_(0, 56, 2), :]
+            xt.noalias(y[:, :, xt.range_(0, 56, 2), :]).assign(
+                xt.clip((vx - params[0]) / xt.sqrt(params[1] + 1e-5) * params[2] + params[3], -3, 3))
+            assert xt.last_kernel() == "Spec_affine_clip_f32"
+            yi = xt.empty(xi.shape, "int32")
+            xt.noalias(yi).assign((xi * 3 + (xi >> 2)) & 0xFFFF)
+            assert xt.last_kernel() == "Spec_bitmix_i32"
+            s = xt.sum(x, [3])
+            r = xt.eval(s)
+            assert xt.last_kernel() == "RSpec_sum_f32"
+            sync()
+            outs.append((y.to_numpy(), yi.to_numpy(), r.to_numpy()))
+    finally:
+        A.lib().xg_debug_nontemporal(1, 1)
+    for u, v in zip(outs[0], outs[1]):
+        assert same_bits(u, v)
+
+
 # ---------------------------------------------------------------- affine + clip through a strided view
 def affine_inputs(n=4, c=256, h=56, w=56, seed=1, dtype=np.float32):
     rng = np.random.default_rng(seed)

@@ -245,3 +245,122 @@ def test_random_reductions(seed):
             if kind == "prod":
                 tol = 1e-5 if dt == "float32" else 1e-12
             assert rel_err(got, hp) <= tol, (kind, dt, shape, axes, k, rel_err(got, hp))
+
+
+# ---------------------------------------------------------------- variance, stddev, minmax, mean<T>
+# variance / stddev: two passes (the inner mean, then the mean of squared
+# deviations), both evaluated in the types xtensor gives them (double for
+# float32 data); bar rel <= 1e-12 against the long-double reference.
+# minmax: exact (min / max are exact; test data has no signed-zero ties).
+_KATS = None
+
+
+def xr_kat(name):
+    import json
+    import os
+    global _KATS
+    if _KATS is None:
+        _KATS = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "kat_arrays.json")))
+    r = _KATS[f"test_extended_xmath_reducers.cpp:xtest_extended_xmath.var_std:{name}"]
+    return np.array(r["values"], np.float64).reshape(r["shape"])
+
+
+def test_variance_stddev_kats_on_device():
+    """test_extended_xmath_reducers.cpp:1824-2833 (numpy goldens, 16 digits)."""
+    a = xt.from_numpy(xr_kat("py_a"))
+    tol = dict(rtol=1e-13, atol=0)
+    np.testing.assert_allclose(run(xt.variance(a))[0][()], xr_kat("py_vr_all"), **tol)
+    np.testing.assert_allclose(run(xt.stddev(a))[0][()], xr_kat("py_st_all"), **tol)
+    np.testing.assert_allclose(run(xt.variance(a, [0, 1, 2, 3], ddof=1))[0][()], xr_kat("py_vr_all_ddof"), **tol)
+    np.testing.assert_allclose(run(xt.variance(a, [0, 2]))[0], xr_kat("py_vr"), **tol)
+    np.testing.assert_allclose(run(xt.variance(a, [0, 2], ddof=1))[0], xr_kat("py_vr_ddof"), **tol)
+    np.testing.assert_allclose(run(xt.stddev(a, [0, 2]))[0], xr_kat("py_st"), **tol)
+
+
+@pytest.mark.parametrize("dtype,shape,axes,ddof", [
+    ("float32", (512, 3000), [0], 0), ("float32", (512, 3000), [1], 1), ("float32", (64, 33, 70), [0, 2], 0),
+    ("float64", (300, 777), None, 1), ("int32", (200, 999), [1], 0), ("float32", (2048, 4096), [1], 0),
+    ("float32", (4096, 2048), [0], 3), ("float32", (7, 1, 5), [1], 0),
+])
+def test_variance_stddev_match_oracle(dtype, shape, axes, ddof):
+    rng = np.random.default_rng(11)
+    if dtype == "int32":
+        h = rng.integers(-1000, 1000, shape, dtype=np.int32)
+    else:
+        h = rng.normal(2.0, 1.5, shape).astype(dtype)
+    a = Dual.of(h)
+    o = oracle_for(a)
+    for rf in (lambda e: xt.variance(e, axes, ddof=ddof), lambda e: xt.stddev(e, axes)):
+        got = xt.eval(rf(a.dev))
+        sync()
+        ref = o.reduce(rf(a.dev), high_precision=True)
+        dt, shp = o.reduce_info(rf(a.dev))
+        assert got.dtype == xt.dtype_name(dt) and tuple(got.shape) == tuple(shp)
+        assert rel_err(got.to_numpy(), ref) <= 1e-12
+
+
+def test_variance_keep_dims_and_bad_ddof():
+    h = np.random.default_rng(3).normal(size=(6, 10))
+    a = xt.from_numpy(h)
+    r, _ = run(xt.variance(a, [1], keep_dims=True))
+    assert r.shape == (6, 1)
+    np.testing.assert_allclose(r[:, 0], h.var(axis=1), rtol=1e-13)
+    with pytest.raises(Exception):
+        run(xt.variance(a, [1], ddof=61))
+    with pytest.raises(Exception):
+        xt.eval(xt.Reducer("sum", a, [1], ddof=1))
+
+
+@pytest.mark.parametrize("dtype,shape", [("float32", (1000, 1000)), ("float64", (333, 77)), ("int32", (4096, 513)),
+                                         ("uint8", (100, 37)), ("float32", (8192, 4096)), ("int16", (3,))])
+def test_minmax_matches_oracle(dtype, shape):
+    rng = np.random.default_rng(5)
+    if dtype.startswith("float"):
+        h = rng.normal(0, 10, shape).astype(dtype)
+        h[np.abs(h) < 1e-3] = 1.0  # no signed-zero ties
+        if h.size > 10:
+            h.flat[rng.integers(0, h.size, 7)] = np.nan  # std::min / std::max skip NaN
+    else:
+        info = np.iinfo(dtype)
+        h = rng.integers(info.min, info.max, shape, dtype=dtype, endpoint=True)
+    a = Dual.of(h)
+    got, _ = run(xt.minmax(a.dev))
+    ref = oracle_for(a).reduce(xt.minmax(a.dev))
+    assert got.dtype == np.dtype(dtype) and got.shape == (2,)
+    assert np.array_equal(got, ref)
+    if dtype.startswith("float"):
+        assert got[0] == np.nanmin(h) and got[1] == np.nanmax(h)
+
+
+def test_minmax_empty_and_expression():
+    e = xt.from_numpy(np.zeros((0, 4), np.float32))
+    got, _ = run(xt.minmax(e))
+    assert got[0] == np.finfo(np.float32).max and got[1] == -np.finfo(np.float32).max
+    h = np.random.default_rng(9).normal(size=(300, 400))
+    a = Dual.of(h)
+    got, _ = run(xt.minmax(xt.exp(a.dev) - 2.0 * a.dev))
+    ref = oracle_for(a).reduce(xt.minmax(xt.exp(a.dev) - 2.0 * a.dev))
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("shape,axes,acc", [((300, 1001), [1], "int64"), ((64, 64), None, "int32"),
+                                            ((1000, 37), [0], "int64"), ((5, 7, 9), [0, 2], "int32")])
+def test_mean_integer_accumulator_divides_in_integers(shape, axes, acc):
+    """core/xmath.hpp:1827-1873: mean<T> with integral T divides by T(n) in integers."""
+    h = np.random.default_rng(2).integers(-5000, 5000, shape, dtype=np.int32)
+    a = Dual.of(h)
+    r = xt.mean(a.dev, axes, dtype=acc)
+    got, _ = run(r)
+    ref = oracle_for(a).reduce(r)
+    assert got.dtype == ref.dtype == np.dtype(acc)
+    assert np.array_equal(got, ref)
+
+
+def test_mean_float_accumulator():
+    h = np.random.default_rng(4).normal(1, 1, (777, 1234)).astype(np.float32)
+    a = Dual.of(h)
+    r = xt.mean(a.dev, [1], dtype="float32")
+    got, _ = run(r)
+    assert got.dtype == np.float32
+    ref = oracle_for(a).reduce(r, high_precision=True)
+    assert rel_err(got, ref) <= 2e-7

@@ -361,7 +361,71 @@ RESULT_TYPES = [
     # templated accumulator: sum<T>(a) has type decltype(T() + value_type()) (core/xmath.hpp:1790-1803)
     ("int32", lambda a: xt.sum(a, dtype="int64"), "int64"), ("float32", lambda a: xt.sum(a, dtype="float64"), "float64"),
     ("int8", lambda a: xt.sum(a, dtype="int8"), "int32"), ("int32", lambda a: xt.prod(a, dtype="float32"), "float32"),
+    # minmax(auchar) is std::array<unsigned char, 2> (test_xmath_result_type.cpp:129)
+    ("uint8", lambda a: xt.minmax(a), "uint8"), ("float32", lambda a: xt.minmax(a), "float32"),
+    # mean<T>, variance<T>, stddev<T> (CHECK_TEMPLATED_RESULT_TYPE_FOR_ALL,
+    # test_xmath_result_type.cpp:72-114): the division is in T's type
+    # (mean_division's value_type = T), sqrt of an integer is double
+    ("uint8", lambda a: xt.mean(a, dtype="uint8"), "int32"), ("int32", lambda a: xt.mean(a, dtype="int64"), "int64"),
+    ("float32", lambda a: xt.mean(a, dtype="float32"), "float32"), ("uint8", lambda a: xt.mean(a, dtype="float32"), "float32"),
+    ("uint8", lambda a: xt.variance(a), "float64"), ("float32", lambda a: xt.variance(a), "float64"),
+    ("uint8", lambda a: xt.variance(a, dtype="uint8"), "int32"), ("uint8", lambda a: xt.stddev(a, dtype="uint8"), "float64"),
+    ("float32", lambda a: xt.stddev(a, dtype="uint8"), "float32"), ("float32", lambda a: xt.variance(a, dtype="float32"), "float32"),
+    ("float64", lambda a: xt.stddev(a), "float64"),
 ]
+
+
+XR = "test_extended_xmath_reducers.cpp"
+
+
+def test_variance_stddev_kats():
+    """test_extended_xmath_reducers.cpp:1824-2833 (numpy goldens printed with
+    16 significant digits): variance / stddev over all axes, over {0, 2}, and
+    with ddof = 1."""
+    T = "xtest_extended_xmath.var_std"
+    a = host(kat(XR, T, "py_a"))
+    o = oracle_for(a)
+    tol = dict(rtol=1e-14, atol=0)
+    np.testing.assert_allclose(o.reduce(xt.variance(a.dev))[()], kat(XR, T, "py_vr_all"), **tol)
+    np.testing.assert_allclose(o.reduce(xt.stddev(a.dev))[()], kat(XR, T, "py_st_all"), **tol)
+    np.testing.assert_allclose(o.reduce(xt.variance(a.dev, [0, 1, 2, 3], ddof=1))[()], kat(XR, T, "py_vr_all_ddof"),
+                               **tol)
+    np.testing.assert_allclose(o.reduce(xt.variance(a.dev, [0, 2])), kat(XR, T, "py_vr"), **tol)
+    np.testing.assert_allclose(o.reduce(xt.variance(a.dev, [0, 2], ddof=1)), kat(XR, T, "py_vr_ddof"), **tol)
+    np.testing.assert_allclose(o.reduce(xt.stddev(a.dev, [0, 2])), kat(XR, T, "py_st"), **tol)
+    # the divisor is (size - ddof) / outputs in size_t arithmetic (detail::mean,
+    # core/xmath.hpp:1836-1850): 840 elements, 35 outputs, ddof 2 -> 838 / 35 = 23
+    # (numpy's n - ddof would be 22)
+    v0 = o.reduce(xt.variance(a.dev, [0, 2]))
+    np.testing.assert_allclose(o.reduce(xt.variance(a.dev, [0, 2], ddof=2)), v0 * 24 / 23, rtol=1e-14)
+
+
+def test_minmax():
+    """core/xmath.hpp:2195-2230 restated: std::min / std::max folds from
+    {max(), lowest()} of the value type; a NaN never replaces the running
+    value (std::min(r, v) = v < r ? v : r)."""
+    f = host(np.array([[3.0, np.nan], [-1.0, 2.0]]))
+    assert np.array_equal(oracle_for(f).reduce(xt.minmax(f.dev)), [-1.0, 3.0])
+    i = host(np.array([7, -4, 9, 0], np.int32))
+    assert np.array_equal(oracle_for(i).reduce(xt.minmax(i.dev)), [-4, 9])
+    e = host(np.zeros((0, 3), np.uint8))
+    assert np.array_equal(oracle_for(e).reduce(xt.minmax(e.dev)), [255, 0])
+    n = host(np.array([np.nan, np.nan], np.float32))
+    r = oracle_for(n).reduce(xt.minmax(n.dev))
+    assert r[0] == np.finfo(np.float32).max and r[1] == -np.finfo(np.float32).max
+
+
+def test_mean_with_accumulator_type():
+    """core/xmath.hpp:1827-1873: mean<T> divides by static_cast<T>(n), so an
+    integral T divides in integers (truncating) and float T in float."""
+    a = host(np.array([[1, 2], [3, 4]], np.int32))
+    o = oracle_for(a)
+    assert o.reduce(xt.mean(a.dev, dtype="int64"))[()] == 2          # 10 / 4
+    assert np.array_equal(o.reduce(xt.mean(a.dev, [1], dtype="int32")), [1, 3])  # 3 / 2, 7 / 2
+    assert np.array_equal(o.reduce(xt.mean(a.dev, [1])), [1.5, 3.5])
+    f = host(np.array([1.0, 2.0, 4.0], np.float32))
+    r = oracle_for(f).reduce(xt.mean(f.dev, dtype="float32"))
+    assert r.dtype == np.float32 and r[()] == np.float32(7.0) / np.float32(3.0)
 
 
 @pytest.mark.parametrize("case", range(len(RESULT_TYPES)))

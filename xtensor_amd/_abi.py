@@ -45,7 +45,8 @@ OP = dict(
 )
 
 # reductions / scans
-RED = dict(sum=0, prod=1, amax=2, amin=3, mean=4, norm_sq=5, norm_l2=6, norm_l1=7, norm_linf=8, norm_l0=9)
+RED = dict(sum=0, prod=1, amax=2, amin=3, mean=4, norm_sq=5, norm_l2=6, norm_l1=7, norm_linf=8, norm_l0=9,
+           variance=10, stddev=11, minmax=12)
 SCAN = dict(cumsum=0, cumprod=1)
 AXIS_NONE = -1000
 
@@ -95,7 +96,7 @@ class ReduceDesc(C.Structure):
         ("has_initial", C.c_int32),
         ("initial", Scalar),
         ("flags", C.c_int32),
-        ("pad_", C.c_int32),
+        ("ddof", C.c_int32),
     ]
 
 
@@ -179,6 +180,7 @@ DEBUG_EXPORTS = {
     "xg_last_kernel": (C.c_char_p, []),
     "xg_debug_force_dynamic": (None, [C.c_int]),
     "xg_debug_units_per_thread": (None, [C.c_int]),
+    "xg_debug_nontemporal": (None, [C.c_int, C.c_int]),
     "xg_debug_assign_kernel": (C.c_int, [_P(Expr), _P(Tensor), C.c_char_p, C.c_int32]),
 }
 

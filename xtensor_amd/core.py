@@ -516,8 +516,9 @@ def cast(e, dtype) -> Func:
 class Reducer(Expr):
     """Lazy reduction; evaluated by `eval`, assignment, or use in an expression."""
 
-    def __init__(self, kind, e, axes=None, keep_dims=False, initial=None, dtype=None):
+    def __init__(self, kind, e, axes=None, keep_dims=False, initial=None, dtype=None, ddof=0):
         self.kind = kind
+        self.ddof = int(ddof)
         self.expr = _wrap(e)
         if axes is not None and isinstance(axes, (numbers.Integral, np.integer)):
             axes = [int(axes)]
@@ -542,6 +543,7 @@ class Reducer(Expr):
         if self.initial is not None:
             d.has_initial = 1
             d.initial = self.initial.to_abi()
+        d.ddof = self.ddof
         return d
 
     def _info(self):
@@ -558,6 +560,23 @@ def _reducer(kind):
 
 for _k in A.RED:
     globals()[_k] = _reducer(_k)
+
+
+def variance(e, axes=None, ddof=0, keep_dims=False, dtype=None):
+    """xt::variance<T>(e, axes, ddof) (core/xmath.hpp:2031-2113): mean<T> of
+    square(e - mean<T>(e, axes)), divided by (size - ddof) / outputs."""
+    return Reducer("variance", e, axes, keep_dims, None, dtype, ddof)
+
+
+def stddev(e, axes=None, keep_dims=False, dtype=None):
+    """xt::stddev<T>(e, axes) = sqrt(variance<T>(e, axes)) (core/xmath.hpp:2046, 2143)."""
+    return Reducer("stddev", e, axes, keep_dims, None, dtype)
+
+
+def minmax(e):
+    """xt::minmax(e) (core/xmath.hpp:2195-2230): [min, max] of every element,
+    folded with std::min / std::max from {max(), lowest()} of the value type."""
+    return Reducer("minmax", e)
 
 
 # ----------------------------------------------------------------------------

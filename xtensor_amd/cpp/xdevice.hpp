@@ -495,7 +495,24 @@ XG_DEV_REDUCER(norm_sq, XG_RED_NORM_SQ)
 XG_DEV_REDUCER(norm_l2, XG_RED_NORM_L2)
 XG_DEV_REDUCER(norm_l1, XG_RED_NORM_L1)
 XG_DEV_REDUCER(norm_linf, XG_RED_NORM_LINF)
+XG_DEV_REDUCER(variance, XG_RED_VARIANCE)
+XG_DEV_REDUCER(stddev, XG_RED_STDDEV)
 #undef XG_DEV_REDUCER
+
+// xt::variance<T>(e, axes, ddof) (core/xmath.hpp:2082-2103): the divisor is
+// (size - ddof) / outputs
+template <class T = void, class E, std::enable_if_t<is_dev_expr<E>::value, int> = 0>
+auto variance(const E& e, std::initializer_list<int> axes, int ddof) {
+    auto r = detail::make_reducer(XG_RED_VARIANCE, e, axes, false, acc_code<T>(), false);
+    r.desc.ddof = ddof;
+    return r;
+}
+
+// xt::minmax(e) (core/xmath.hpp:2195-2230): {min, max} of every element as a
+// [2] result of the value type
+template <class E, std::enable_if_t<is_dev_expr<E>::value, int> = 0> auto minmax(const E& e) {
+    return detail::make_reducer(XG_RED_MINMAX, e, {}, false, -1, true);
+}
 
 // ---------------------------------------------------------------- evaluation
 template <class E> xg_expr flatten_expr(const E& e, descriptor& d) {

@@ -107,8 +107,12 @@ template <int AK, int OP> struct Acc {
             else return a * b;
         } else if constexpr (OP == RO_MAX) {
             return a > b ? a : b;  // math::maximum: select(a > b, a, b)
-        } else {
+        } else if constexpr (OP == RO_MIN) {
             return a < b ? a : b;  // math::minimum: select(a < b, a, b)
+        } else if constexpr (OP == RO_MAXNUM) {
+            return a < b ? b : a;  // std::max(a, b)
+        } else {
+            return b < a ? b : a;  // std::min(a, b)
         }
     }
     __device__ void init(uint64_t id) { v = reg_get<T>(id); }

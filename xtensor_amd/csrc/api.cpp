@@ -112,6 +112,119 @@ EwPlan xg::plan_assign_best(const xg_expr* e, const xg_tensor* out, int* V) {
     return p;
 }
 
+// plan with the compiled-kernel vector width; without a matching compiled
+// program, re-plan for the generic kernels (V = 4)
+RedPlan xg::plan_reduce_best(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, int* V) {
+    const int cus = device_cus();
+    *V = choose_vector_width(e, nullptr);
+    RedPlan p = plan_reduce(e, r, out, *V, cus, nullptr, nullptr);
+    if (*V != 4 && !find_rspec(p, *V)) {
+        *V = 4;
+        p = plan_reduce(e, r, out, *V, cus, nullptr, nullptr);
+    }
+    return p;
+}
+
+namespace {
+
+// xt::variance / xt::stddev (core/xmath.hpp:2031-2113): the inner
+// mean<T>(e, axes) is evaluated first into a kept-dims temporary m, then
+// mean<T>(square(e - m), axes, ddof) -- two passes over e, as in xtensor.
+// m and the split partials share the stream's workspace.
+void reduce_moment(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t st) {
+    if (!e || !e->nodes || e->n_nodes < 1) raise(XG_INVALID_ARGUMENT, "empty expression");
+    xg_reduce_desc r1 = *r;
+    r1.op = XG_RED_MEAN;
+    r1.keep_dims = 1;
+    r1.has_initial = 0;
+    r1.ddof = 0;
+    int mdt = 0;
+    Shape ms;
+    reduce_result(e, &r1, &mdt, &ms);
+    const int64_t nm = numel(ms);
+    const size_t mbytes = (size_t(std::max<int64_t>(nm, 1)) * size_t(dtype_size(mdt)) + 255) & ~size_t(255);
+    xg_tensor m{};
+    m.dtype = mdt;
+    m.ndim = int32_t(ms.size());
+    int64_t stv = 1;
+    for (int k = m.ndim - 1; k >= 0; --k) {
+        m.shape[k] = ms[size_t(k)];
+        m.strides[k] = stv;
+        stv *= std::max<int64_t>(ms[size_t(k)], 1);
+    }
+    // e - m: the tree of e, m as one more leaf, and the subtraction as root
+    std::vector<xg_node> nodes(e->nodes, e->nodes + e->n_nodes);
+    std::vector<xg_tensor> leaves(e->leaves, e->leaves + e->n_leaves);
+    leaves.push_back(m);
+    xg_node ln{};
+    ln.op = XG_OP_LEAF;
+    ln.index = e->n_leaves;
+    nodes.push_back(ln);
+    xg_node sub{};
+    sub.op = XG_OP_MINUS;
+    sub.nargs = 2;
+    sub.args[0] = e->n_nodes - 1;
+    sub.args[1] = e->n_nodes;
+    nodes.push_back(sub);
+    xg_expr e2 = *e;
+    e2.n_nodes = int32_t(nodes.size());
+    e2.n_leaves = int32_t(leaves.size());
+    e2.nodes = nodes.data();
+    e2.leaves = leaves.data();
+    // plan with a placeholder address to size the workspace, then for real
+    auto plan_both = [&](char* base, RedPlan* p1, int* V1, RedPlan* p2, int* V2) {
+        m.data = base;
+        leaves.back().data = base;
+        *p1 = plan_reduce_best(e, &r1, &m, V1);
+        *p2 = plan_reduce_best(&e2, r, out, V2);
+    };
+    RedPlan p1, p2;
+    int V1 = 4, V2 = 4;
+    plan_both(reinterpret_cast<char*>(uintptr_t(1) << 40), &p1, &V1, &p2, &V2);
+    if (p2.n_out == 0) {
+        note_kernel("empty");
+        return;
+    }
+    const size_t wbytes = std::max(reduce_workspace_bytes(p1), reduce_workspace_bytes(p2));
+    char* base = static_cast<char*>(workspace(mbytes + wbytes, st));
+    plan_both(base, &p1, &V1, &p2, &V2);
+    const char* used = nullptr;
+    if (p1.n_out) {
+        p1.partials = wbytes ? base + mbytes : nullptr;
+        check_hip(launch_reduce(p1, V1, st, &used), "variance mean launch");
+    }
+    p2.partials = wbytes ? base + mbytes : nullptr;
+    check_hip(launch_reduce(p2, V2, st, &used), "variance launch");
+    note_kernel(used);
+}
+
+// xt::minmax (core/xmath.hpp:2195-2230): out[0] = the std::min fold,
+// out[1] = the std::max fold over every element, from max() / lowest().
+void reduce_minmax(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t st) {
+    int dt = 0;
+    Shape rs;
+    reduce_result(e, r, &dt, &rs);
+    if (!out || out->ndim != 1 || out->shape[0] != 2)
+        raise(XG_BROADCAST_ERROR, "minmax output must have shape (2)");
+    const char* used = nullptr;
+    for (int k = 0; k < 2; ++k) {
+        xg_reduce_desc rk{};
+        rk.op = k == 0 ? kRedMinNum : kRedMaxNum;
+        rk.acc_dtype = -1;
+        rk.n_axes = -1;
+        xg_tensor ok = *out;
+        ok.ndim = 0;
+        ok.offset = out->offset + int64_t(k) * out->strides[0];
+        int V = 4;
+        RedPlan p = plan_reduce_best(e, &rk, &ok, &V);
+        p.partials = workspace(reduce_workspace_bytes(p), st);
+        check_hip(launch_reduce(p, V, st, &used), "minmax launch");
+    }
+    note_kernel(used);
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* xg_version(void) { return "xtensor_amd 0.1.0 (gfx950)"; }
@@ -165,13 +278,16 @@ xg_status xg_reduce_info(const xg_expr* e, const xg_reduce_desc* r, int32_t* dty
 xg_status xg_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, xg_stream_t stream) {
     return guard([&] {
         hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-        int cus = device_cus();
-        int V = choose_vector_width(e, nullptr);
-        RedPlan p = plan_reduce(e, r, out, V, cus, nullptr, nullptr);
-        if (V != 4 && !find_rspec(p, V)) {  // the generic kernels are built for V = 4
-            V = 4;
-            p = plan_reduce(e, r, out, V, cus, nullptr, nullptr);
+        if (r && (r->op == XG_RED_VARIANCE || r->op == XG_RED_STDDEV)) {
+            reduce_moment(e, r, out, st);
+            return;
         }
+        if (r && r->op == XG_RED_MINMAX) {
+            reduce_minmax(e, r, out, st);
+            return;
+        }
+        int V = 4;
+        RedPlan p = plan_reduce_best(e, r, out, &V);
         if (p.n_out == 0) {
             note_kernel("empty");
             return;
@@ -287,13 +403,9 @@ xg_status xg_plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_ten
         auto* p = new xg_plan_s();
         try {
             p->kind = 1;
-            int cus = device_cus();
-            p->V = choose_vector_width(e, nullptr);
-            p->red = plan_reduce(e, r, out, p->V, cus, nullptr, nullptr);
-            if (p->V != 4 && !find_rspec(p->red, p->V)) {
-                p->V = 4;
-                p->red = plan_reduce(e, r, out, p->V, cus, nullptr, nullptr);
-            }
+            if (r && (r->op == XG_RED_VARIANCE || r->op == XG_RED_STDDEV || r->op == XG_RED_MINMAX))
+                raise(XG_UNSUPPORTED, "variance, stddev and minmax run several passes; use xg_reduce");
+            p->red = plan_reduce_best(e, r, out, &p->V);
             p->ws = reduce_workspace_bytes(p->red);
         } catch (...) {
             delete p;
@@ -377,6 +489,11 @@ void xg_debug_force_dynamic(int on) { ew_set_force_dynamic(on); }
 // Measurement aid: 1 or 4 work units per thread in compiled elementwise
 // kernels, 0 = chosen by size.
 void xg_debug_units_per_thread(int upt) { ew_set_units_per_thread(upt); }
+
+// Measurement aid: non-temporal loads of large streaming leaves and
+// non-temporal stores of large outputs (both on by default), for plans made
+// after the call.
+void xg_debug_nontemporal(int loads, int stores) { set_nontemporal(loads, stores); }
 
 // Name of the compiled specialisation an assignment would use, "" if none.
 xg_status xg_debug_assign_kernel(const xg_expr* e, const xg_tensor* out, char* name, int32_t cap) {

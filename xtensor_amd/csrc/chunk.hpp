@@ -33,16 +33,25 @@ __device__ __forceinline__ void load_leaf(const LeafView& lv, int mode, int dt, 
     const int esz = dtype_size(dt);
     auto off = unit_offset<I64>(lv.stride, idx, rank, j0);
     const char* p = lv.ptr + int64_t(off) * esz;
+    const bool nt = lm_nt(mode);
+    mode = lm_base(mode);
     if (mode == LM_BCAST) {
         Reg x = from_storage(load_raw(p, esz), dt);
 #pragma unroll
         for (int v = 0; v < V; ++v) out[v] = x;
     } else if (mode == LM_CONTIG && nvalid == V) {
-        load_vec<V>(p, esz, dt, out);
+        if (nt) load_vec<V, true>(p, esz, dt, out);
+        else load_vec<V>(p, esz, dt, out);
     } else {
         const int64_t is = lv.stride[rank - 1];
+        if (nt) {
 #pragma unroll
-        for (int v = 0; v < V; ++v) out[v] = v < nvalid ? from_storage(load_raw(p + int64_t(v) * is * esz, esz), dt) : 0;
+            for (int v = 0; v < V; ++v)
+                out[v] = v < nvalid ? from_storage(load_raw<true>(p + int64_t(v) * is * esz, esz), dt) : 0;
+        } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v) out[v] = v < nvalid ? from_storage(load_raw(p + int64_t(v) * is * esz, esz), dt) : 0;
+        }
     }
 }
 
@@ -53,7 +62,7 @@ __device__ __forceinline__ void run_static(const LeafView* leaves, const uint64_
     if constexpr (I < S::nops) {
         constexpr MicroOp m = S::ops[I];
         if constexpr (m.op == MO_LOAD) {
-            load_leaf<V, I64>(leaves[m.aux], S::leaf_mode[m.aux], S::leaf_dtype[m.aux], idx, rank, j0, nvalid,
+            load_leaf<V, I64>(leaves[m.aux], S::leaf_mode[m.aux] | (leaves[m.aux].mode & LM_NT), S::leaf_dtype[m.aux], idx, rank, j0, nvalid,
                               R[m.dst]);
         } else {
             Reg tmp[V];

@@ -254,17 +254,27 @@ __device__ __forceinline__ Reg from_storage(uint64_t raw, int dt) {
     }
 }
 
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
+// one load of type T; NT = non-temporal (streaming) cache policy
+template <bool NT, class T> __device__ __forceinline__ T ld(const T* p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+template <bool NT = false>
 __device__ __forceinline__ uint64_t load_raw(const char* p, int esz) {
     switch (esz) {
-    case 1: return *reinterpret_cast<const uint8_t*>(p);
-    case 2: return *reinterpret_cast<const uint16_t*>(p);
-    case 4: return *reinterpret_cast<const uint32_t*>(p);
-    default: return *reinterpret_cast<const uint64_t*>(p);
+    case 1: return ld<NT>(reinterpret_cast<const uint8_t*>(p));
+    case 2: return ld<NT>(reinterpret_cast<const uint16_t*>(p));
+    case 4: return ld<NT>(reinterpret_cast<const uint32_t*>(p));
+    default: return ld<NT>(reinterpret_cast<const uint64_t*>(p));
     }
 }
 
 // V consecutive, naturally aligned elements of size esz in as few wide loads as possible
-template <int V>
+template <int V, bool NT = false>
 __device__ __forceinline__ void load_vec(const char* p, int esz, int dt, Reg (&out)[V]) {
     const int bytes = V * esz;
     if (bytes >= 16) {
@@ -273,7 +283,7 @@ __device__ __forceinline__ void load_vec(const char* p, int esz, int dt, Reg (&o
 #pragma unroll
         for (int q = 0; q < MAXW / 4; ++q) {
             if (q * 16 < bytes) {
-                uint4 x = reinterpret_cast<const uint4*>(p)[q];
+                u32x4_t x = ld<NT>(reinterpret_cast<const u32x4_t*>(p) + q);
                 w[4 * q] = x.x; w[4 * q + 1] = x.y; w[4 * q + 2] = x.z; w[4 * q + 3] = x.w;
             }
         }
@@ -287,17 +297,17 @@ __device__ __forceinline__ void load_vec(const char* p, int esz, int dt, Reg (&o
             out[v] = from_storage(raw, dt);
         }
     } else if (bytes == 8) {
-        uint2 x = *reinterpret_cast<const uint2*>(p);
+        u32x2_t x = ld<NT>(reinterpret_cast<const u32x2_t*>(p));
         uint64_t all = uint64_t(x.x) | (uint64_t(x.y) << 32);
 #pragma unroll
         for (int v = 0; v < V; ++v) out[v] = from_storage(all >> (v * esz * 8), dt);
     } else if (bytes == 4) {
-        uint32_t all = *reinterpret_cast<const uint32_t*>(p);
+        uint32_t all = ld<NT>(reinterpret_cast<const uint32_t*>(p));
 #pragma unroll
         for (int v = 0; v < V; ++v) out[v] = from_storage(all >> (v * esz * 8), dt);
     } else {
 #pragma unroll
-        for (int v = 0; v < V; ++v) out[v] = from_storage(load_raw(p + v * esz, esz), dt);
+        for (int v = 0; v < V; ++v) out[v] = from_storage(load_raw<NT>(p + v * esz, esz), dt);
     }
 }
 
@@ -311,16 +321,23 @@ __device__ __forceinline__ uint64_t to_storage(Reg r, int dt) {
     return r;
 }
 
+// one store of type T; NT = non-temporal (streaming) cache policy
+template <bool NT, class T> __device__ __forceinline__ void st(T* p, T v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <bool NT = false>
 __device__ __forceinline__ void store_raw(char* p, int esz, uint64_t raw) {
     switch (esz) {
-    case 1: *reinterpret_cast<uint8_t*>(p) = uint8_t(raw); break;
-    case 2: *reinterpret_cast<uint16_t*>(p) = uint16_t(raw); break;
-    case 4: *reinterpret_cast<uint32_t*>(p) = uint32_t(raw); break;
-    default: *reinterpret_cast<uint64_t*>(p) = raw; break;
+    case 1: st<NT>(reinterpret_cast<uint8_t*>(p), uint8_t(raw)); break;
+    case 2: st<NT>(reinterpret_cast<uint16_t*>(p), uint16_t(raw)); break;
+    case 4: st<NT>(reinterpret_cast<uint32_t*>(p), uint32_t(raw)); break;
+    default: st<NT>(reinterpret_cast<uint64_t*>(p), raw); break;
     }
 }
 
-template <int V>
+template <int V, bool NT = false>
 __device__ __forceinline__ void store_vec(char* p, int esz, int dt, const Reg (&in)[V]) {
     const int bytes = V * esz;
     if (bytes >= 16) {
@@ -338,21 +355,24 @@ __device__ __forceinline__ void store_vec(char* p, int esz, int dt, const Reg (&
         }
 #pragma unroll
         for (int q = 0; q < MAXW / 4; ++q)
-            if (q * 16 < bytes)
-                reinterpret_cast<uint4*>(p)[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+            if (q * 16 < bytes) {
+                u32x4_t x = {w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]};
+                st<NT>(reinterpret_cast<u32x4_t*>(p) + q, x);
+            }
     } else if (bytes == 8) {
         uint64_t all = 0;
 #pragma unroll
         for (int v = 0; v < V; ++v) all |= (to_storage(in[v], dt) & ((1ull << (esz * 8)) - 1)) << (v * esz * 8);
-        *reinterpret_cast<uint2*>(p) = make_uint2(uint32_t(all), uint32_t(all >> 32));
+        u32x2_t x = {uint32_t(all), uint32_t(all >> 32)};
+        st<NT>(reinterpret_cast<u32x2_t*>(p), x);
     } else if (bytes == 4) {
         uint32_t all = 0;
 #pragma unroll
         for (int v = 0; v < V; ++v) all |= (uint32_t(to_storage(in[v], dt)) & ((1u << (esz * 8)) - 1)) << (v * esz * 8);
-        *reinterpret_cast<uint32_t*>(p) = all;
+        st<NT>(reinterpret_cast<uint32_t*>(p), all);
     } else {
 #pragma unroll
-        for (int v = 0; v < V; ++v) store_raw(p + v * esz, esz, to_storage(in[v], dt));
+        for (int v = 0; v < V; ++v) store_raw<NT>(p + v * esz, esz, to_storage(in[v], dt));
     }
 }
 

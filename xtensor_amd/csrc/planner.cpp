@@ -21,6 +21,10 @@
 
 namespace xg {
 
+int g_nt_loads = 1;   // non-temporal loads of streaming leaves
+int g_nt_stores = 1;  // non-temporal stores of large outputs
+
+
 [[noreturn]] void raise(int code, const std::string& msg) { throw Error{code, msg}; }
 
 int64_t numel(const Shape& s) {
@@ -625,6 +629,7 @@ EwPlan plan_assign(const xg_expr* e, const xg_tensor* out, int V) {
         return true;
     };
     p.out_mode = aligned(p.out_ptr, osz, p.out_stride[rank - 1], p.out_stride) ? LM_CONTIG : LM_STRIDED;
+    if (g_nt_stores && int64_t(total) * osz >= kNtMinBytes) p.out_mode |= LM_NT;
     for (int pl = 0; pl < p.prog.nleaves; ++pl) {
         int l = p.prog.leaf_src[pl];
         LeafView& lv = p.leaves[pl];
@@ -640,6 +645,7 @@ EwPlan plan_assign(const xg_expr* e, const xg_tensor* out, int V) {
         max_off = std::max(max_off, lmax);
         int64_t is = lv.stride[rank - 1];
         lv.mode = is == 0 ? LM_BCAST : aligned(lv.ptr, esz, is, lv.stride) ? LM_CONTIG : LM_STRIDED;
+        if (lv.mode != LM_BCAST && streams(lv.stride, p.shape, rank, esz)) lv.mode |= LM_NT;
     }
     p.index64 = max_off >= (int64_t(1) << 31) ? 1 : 0;
     p.compact = (!p.index64 && rank <= 4) ? 1 : 0;
@@ -653,6 +659,21 @@ EwPlan plan_assign(const xg_expr* e, const xg_tensor* out, int V) {
     return p;
 }
 
+void set_nontemporal(int loads, int stores) {
+    g_nt_loads = loads;
+    g_nt_stores = stores;
+}
+
+bool streams(const int64_t* stride, const int64_t* shape, int rank, int esz) {
+    if (!g_nt_loads) return false;
+    int64_t n = 1;
+    for (int k = 0; k < rank; ++k) {
+        if (shape[k] > 1 && stride[k] == 0) return false;
+        n *= std::max<int64_t>(shape[k], 0);
+    }
+    return n * esz >= kNtMinBytes;
+}
+
 SpecKey spec_key(const EwPlan& p) {
     SpecKey k{};
     k.nops = p.prog.nops;
@@ -660,10 +681,10 @@ SpecKey spec_key(const EwPlan& p) {
     k.nleaves = p.prog.nleaves;
     for (int l = 0; l < p.prog.nleaves; ++l) {
         k.leaf_dtype[l] = uint8_t(p.leaves[l].dtype);
-        k.leaf_mode[l] = uint8_t(p.leaves[l].mode);
+        k.leaf_mode[l] = uint8_t(lm_base(p.leaves[l].mode));
     }
     k.out_dtype = uint8_t(p.out_dtype);
-    k.out_mode = uint8_t(p.out_mode);
+    k.out_mode = uint8_t(lm_base(p.out_mode));
     k.result_reg = p.prog.result_reg;
     return k;
 }

@@ -67,6 +67,13 @@ struct SpecKey {
 };
 
 SpecKey spec_key(const EwPlan& p);
+
+// True when a leaf with these element strides over an iteration space of
+// this shape reads every element once (no stride-0 dim of extent > 1) and
+// spans at least kNtMinBytes: its loads then stream non-temporally.
+bool streams(const int64_t* stride, const int64_t* shape, int rank, int esz);
+// Switch non-temporal leaf loads / output stores on or off (measurement aid).
+void set_nontemporal(int loads, int stores);
 bool spec_equal(const SpecKey& a, const SpecKey& b);
 
 int64_t numel(const Shape& s);

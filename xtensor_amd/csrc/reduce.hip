@@ -70,7 +70,7 @@ const RSpecEntry* find_rspec(const RedPlan& p, int V) {
     k.nleaves = p.prog.nleaves;
     for (int l = 0; l < p.prog.nleaves; ++l) {
         k.leaf_dtype[l] = uint8_t(p.leaves[l].dtype);
-        k.leaf_mode[l] = uint8_t(p.leaves[l].mode);
+        k.leaf_mode[l] = uint8_t(lm_base(p.leaves[l].mode));
     }
     k.result_reg = p.prog.result_reg;
     for (auto& s : rspec_table())

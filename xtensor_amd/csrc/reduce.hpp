@@ -26,9 +26,18 @@ enum AccKind : int32_t {
     AK_NUM
 };
 
-enum RedOp : int32_t { RO_ADD = 0, RO_MUL, RO_MAX, RO_MIN, RO_NUM };
+// RO_MAX / RO_MIN fold with xt::maximum / xt::minimum (select(a > b, a, b),
+// a = accumulator: a NaN value enters the accumulator); RO_MAXNUM /
+// RO_MINNUM with std::max / std::min (a NaN value never replaces it), for
+// xt::minmax on floats.
+enum RedOp : int32_t { RO_ADD = 0, RO_MUL, RO_MAX, RO_MIN, RO_MAXNUM, RO_MINNUM, RO_NUM };
 
-enum Finalize : int32_t { FZ_NONE = 0, FZ_MEAN = 1, FZ_SQRT = 2 };
+// FZ_MEAN: value / divisor in the result type; FZ_MEAN_SQRT: sqrt of that
+enum Finalize : int32_t { FZ_NONE = 0, FZ_MEAN = 1, FZ_SQRT = 2, FZ_MEAN_SQRT = 3 };
+
+// internal pass codes of xt::minmax (its min and max folds)
+constexpr int kRedMinNum = 100;
+constexpr int kRedMaxNum = 101;
 
 // waves per workgroup of the compiled cols kernel (they split the rows of
 // one column tile and combine in LDS)
@@ -65,7 +74,9 @@ struct RedPlan {
     int32_t out_dtype;
     uint64_t identity;  // start value of every accumulator, in acc representation
     uint64_t initial;   // xt::initial, in acc representation
-    double mean_div;
+    int32_t mdt;        // FZ_MEAN / FZ_MEAN_SQRT: type of value / divisor
+    int32_t pad0_;
+    uint64_t div_bits;  // the divisor, in register form of mdt
     char* out_ptr;
     int64_t out_stride[kMaxRank];  // over the kept dims in normalised order (fast last for RL_COLS)
     void* partials;     // workspace: splits * n_out accumulators of 16 bytes

@@ -14,6 +14,10 @@ KFn combine_kernel(int ak, int op, bool wide) {
     XG_C(AK_F64, RO_MUL)
     XG_C(AK_F64, RO_MAX)
     XG_C(AK_F64, RO_MIN)
+    XG_C(AK_F32, RO_MAXNUM)
+    XG_C(AK_F32, RO_MINNUM)
+    XG_C(AK_F64, RO_MAXNUM)
+    XG_C(AK_F64, RO_MINNUM)
     XG_C(AK_I32, RO_ADD)
     XG_C(AK_I32, RO_MUL)
     XG_C(AK_I32, RO_MAX)

@@ -13,6 +13,10 @@ bool dyn_reduce_float(int ak, int op, RedKernels* k) {
     XG_D(AK_F64, RO_MUL)
     XG_D(AK_F64, RO_MAX)
     XG_D(AK_F64, RO_MIN)
+    XG_D(AK_F32, RO_MAXNUM)
+    XG_D(AK_F32, RO_MINNUM)
+    XG_D(AK_F64, RO_MAXNUM)
+    XG_D(AK_F64, RO_MINNUM)
 #undef XG_D
     return false;
 }

@@ -29,6 +29,19 @@ __device__ __forceinline__ int64_t out_offset(const RedPlan& p, uint32_t o) {
     return off;
 }
 
+// a / b in value type dt (int32 or wider; integer division truncates, and
+// a zero divisor gives 0 where C++ leaves it undefined)
+__device__ __forceinline__ Reg div_in(Reg a, Reg b, int dt) {
+    switch (dt) {
+    case XG_FLOAT32: return flt_binop<float>(MO_DIV, a, b);
+    case XG_FLOAT64: return flt_binop<double>(MO_DIV, a, b);
+    case XG_INT32: return int_binop<int32_t>(MO_DIV, a, b);
+    case XG_UINT32: return int_binop<uint32_t>(MO_DIV, a, b);
+    case XG_INT64: return int_binop<int64_t>(MO_DIV, a, b);
+    default: return int_binop<uint64_t>(MO_DIV, a, b);
+    }
+}
+
 template <class A> __device__ void finish(const RedPlan& p, A acc, uint32_t o) {
     if (p.has_initial) {  // xreducer merges `initial` in front of the folded value
         A ini;
@@ -38,12 +51,14 @@ template <class A> __device__ void finish(const RedPlan& p, A acc, uint32_t o) {
     }
     Reg v = acc.value(p.vdt);
     Reg r = v;
-    if (p.fin == FZ_MEAN) {
-        double d = as_f64(cvt(v, p.vdt, XG_FLOAT64));
-        r = from_f64(d / p.mean_div);
-    } else if (p.fin == FZ_SQRT) {
-        if (p.rdt == XG_FLOAT32) r = from_f32(sqrtf(as_f32(cvt(v, p.vdt, XG_FLOAT32))));
-        else r = from_f64(sqrt(as_f64(cvt(v, p.vdt, XG_FLOAT64))));
+    int rt = p.vdt;
+    if (p.fin == FZ_MEAN || p.fin == FZ_MEAN_SQRT) {  // s / div in their common type
+        r = div_in(cvt(v, p.vdt, p.mdt), Reg(p.div_bits), p.mdt);
+        rt = p.mdt;
+    }
+    if (p.fin == FZ_SQRT || p.fin == FZ_MEAN_SQRT) {
+        if (p.rdt == XG_FLOAT32) r = from_f32(sqrtf(as_f32(cvt(r, rt, XG_FLOAT32))));
+        else r = from_f64(sqrt(as_f64(cvt(r, rt, XG_FLOAT64))));
     }
     const int odt = p.out_dtype;
     Reg ov = odt == XG_BFLOAT16 ? cvt(r, p.rdt, XG_FLOAT32) : cvt(r, p.rdt, odt);

@@ -36,21 +36,45 @@ struct RedTypes {
     int op;    // RedOp
     int fin;   // Finalize
     int init_dt;
+    int div_dt;  // FZ_MEAN*: the divisor is static_cast to this type (T, or double)
+    int mdt;     // FZ_MEAN*: type of value / divisor
 };
 
+// Types of one reduction pass.  For XG_RED_VARIANCE / XG_RED_STDDEV, `edt`
+// is the type of the deviation e - mean(e) whose squares are averaged.
 RedTypes red_types(int op, int acc_dtype, int edt) {
     RedTypes t{};
-    int it = (acc_dtype >= 0 && op <= XG_RED_MEAN) ? acc_dtype : edt;
+    const bool templ = op <= XG_RED_MEAN || op == XG_RED_VARIANCE || op == XG_RED_STDDEV;
+    int it = (acc_dtype >= 0 && templ) ? acc_dtype : edt;
     if (it == XG_BFLOAT16 || it < 0 || it >= XG_NUM_DTYPES) raise(XG_INVALID_ARGUMENT, "bad accumulator dtype");
     t.init_dt = it;
     t.map = MAP_NONE;
     t.fin = FZ_NONE;
+    // mean_division / mean_noaxis: value_type = T, or double without T (core/xmath.hpp:1827-1873)
+    t.div_dt = acc_dtype >= 0 ? acc_dtype : XG_FLOAT64;
     switch (op) {
     case XG_RED_SUM: t.vdt = bin(XG_OP_PLUS, it, edt); t.op = RO_ADD; break;
     case XG_RED_PROD: t.vdt = bin(XG_OP_MULTIPLIES, it, edt); t.op = RO_MUL; break;
     case XG_RED_AMAX: t.vdt = bin(XG_OP_MAXIMUM, it, edt); t.op = RO_MAX; break;
     case XG_RED_AMIN: t.vdt = bin(XG_OP_MINIMUM, it, edt); t.op = RO_MIN; break;
     case XG_RED_MEAN: t.vdt = bin(XG_OP_PLUS, it, edt); t.op = RO_ADD; t.fin = FZ_MEAN; break;
+    case XG_RED_VARIANCE: case XG_RED_STDDEV: {
+        // detail::mean<T>(square(e - m), axes, ddof): sum<T> of the squares / T-or-double(n)
+        t.map = MAP_SQUARE;
+        int q = mapped_dtype(MAP_SQUARE, edt);
+        int it2 = acc_dtype >= 0 ? acc_dtype : q;
+        t.init_dt = it2;
+        t.vdt = bin(XG_OP_PLUS, it2, q);
+        t.op = RO_ADD;
+        t.fin = op == XG_RED_VARIANCE ? FZ_MEAN : FZ_MEAN_SQRT;
+        break;
+    }
+    case kRedMinNum: case kRedMaxNum:  // the two folds of xt::minmax, in the value type
+        t.vdt = edt;
+        t.init_dt = edt;
+        if (is_float(edt)) t.op = op == kRedMinNum ? RO_MINNUM : RO_MAXNUM;
+        else t.op = op == kRedMinNum ? RO_MIN : RO_MAX;
+        break;
     case XG_RED_NORM_SQ: case XG_RED_NORM_L2:
         t.map = MAP_SQUARE;
         t.vdt = bin(XG_OP_PLUS, big_promote(edt), mapped_dtype(MAP_SQUARE, edt));
@@ -67,12 +91,22 @@ RedTypes red_types(int op, int acc_dtype, int edt) {
     default: raise(XG_INVALID_ARGUMENT, "unknown reduction " + std::to_string(op));
     }
     t.rdt = t.vdt;
-    if (t.fin == FZ_MEAN) t.rdt = bin(XG_OP_DIVIDES, t.vdt, XG_FLOAT64);
-    if (t.fin == FZ_SQRT) {
-        int at[1] = {t.vdt};
+    if (t.fin == FZ_MEAN || t.fin == FZ_MEAN_SQRT) {
+        if (t.div_dt == XG_BFLOAT16) raise(XG_INVALID_ARGUMENT, "bad accumulator dtype");
+        t.mdt = bin(XG_OP_DIVIDES, t.vdt, t.div_dt);
+        t.rdt = t.mdt;
+    }
+    if (t.fin == FZ_SQRT || t.fin == FZ_MEAN_SQRT) {
+        int at[1] = {t.rdt};
         t.rdt = result_dtype(XG_OP_SQRT, 1, at, 0);
     }
     return t;
+}
+
+// the type of e - mean<T>(e) for an element type edt (variance / stddev)
+int deviation_dtype(int acc_dtype, int edt) {
+    RedTypes m = red_types(XG_RED_MEAN, acc_dtype, edt);
+    return bin(XG_OP_MINUS, edt, m.rdt);
 }
 
 std::vector<int> norm_axes(const xg_reduce_desc* r, int ndim) {
@@ -179,16 +213,20 @@ xg_scalar limit_scalar(int dt, bool lowest) {
     return s;
 }
 
+// the value held in register bits `vb` of type dt, as a scalar
+xg_scalar scalar_of_bits(uint64_t vb, int dt) {
+    xg_scalar sv{};
+    sv.dtype = dt;
+    if (dt == XG_FLOAT32) { float x; uint32_t b = uint32_t(vb); std::memcpy(&x, &b, 4); sv.v.f = x; }
+    else if (dt == XG_FLOAT64) std::memcpy(&sv.v.f, &vb, 8);
+    else if (is_unsigned_int(dt) && dt != XG_BOOL) sv.v.u = (dt == XG_UINT64) ? vb : uint64_t(uint32_t(vb));
+    else sv.v.i = (dt == XG_INT64) ? int64_t(vb) : int64_t(int32_t(uint32_t(vb)));
+    return sv;
+}
+
 // value -> V -> accumulator representation (identity / initial)
 uint64_t to_acc_bits(const xg_scalar& s, int vdt, int ak) {
-    uint64_t vb = scalar_bits(s, vdt);
-    xg_scalar sv{};
-    sv.dtype = vdt;
-    if (vdt == XG_FLOAT32) { float x; uint32_t b = uint32_t(vb); std::memcpy(&x, &b, 4); sv.v.f = x; }
-    else if (vdt == XG_FLOAT64) std::memcpy(&sv.v.f, &vb, 8);
-    else if (is_unsigned_int(vdt) && vdt != XG_BOOL) sv.v.u = (vdt == XG_UINT64) ? vb : uint64_t(uint32_t(vb));
-    else sv.v.i = (vdt == XG_INT64) ? int64_t(vb) : int64_t(int32_t(uint32_t(vb)));
-    return scalar_bits(sv, acc_input_dtype(ak));
+    return scalar_bits(scalar_of_bits(scalar_bits(s, vdt), vdt), acc_input_dtype(ak));
 }
 
 struct RDim {
@@ -232,8 +270,17 @@ uint32_t pow2_ceil(uint64_t x) {
 void reduce_result(const xg_expr* e, const xg_reduce_desc* r, int* dtype, Shape* shape) {
     if (!r) raise(XG_INVALID_ARGUMENT, "null reduce descriptor");
     ExprInfo info = analyse(e);
+    if (r->op == XG_RED_MINMAX) {  // std::array<value_type, 2> over every element
+        if (r->n_axes >= 0 || r->keep_dims || r->has_initial)
+            raise(XG_INVALID_ARGUMENT, "minmax reduces over all axes and takes no options");
+        *dtype = info.dtype;
+        *shape = Shape{2};
+        return;
+    }
     std::vector<int> ax = norm_axes(r, int(info.shape.size()));
-    RedTypes t = red_types(r->op, r->acc_dtype, info.dtype);
+    int edt = info.dtype;
+    if (r->op == XG_RED_VARIANCE || r->op == XG_RED_STDDEV) edt = deviation_dtype(r->acc_dtype, edt);
+    RedTypes t = red_types(r->op, r->acc_dtype, edt);
     Shape rs;
     for (int k = 0; k < int(info.shape.size()); ++k) {
         bool red = std::find(ax.begin(), ax.end(), k) != ax.end();
@@ -252,6 +299,9 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
     ExprInfo info = analyse(e);
     const Shape& in = info.shape;
     int nd = int(in.size());
+    if (r->op == XG_RED_MINMAX) raise(XG_INVALID_ARGUMENT, "minmax is planned as two folds");
+    const bool moment = r->op == XG_RED_VARIANCE || r->op == XG_RED_STDDEV;
+    if (r->ddof != 0 && !moment) raise(XG_INVALID_ARGUMENT, "ddof applies to variance and stddev only");
     std::vector<int> ax = norm_axes(r, nd);
     RedTypes t = red_types(r->op, r->acc_dtype, info.dtype);
     Shape rs;
@@ -281,7 +331,7 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
         init.dtype = XG_INT32;
         init.v.i = t.op == RO_MUL ? 1 : 0;
     } else {
-        init = limit_scalar(t.init_dt, t.op == RO_MAX);
+        init = limit_scalar(t.init_dt, t.op == RO_MAX || t.op == RO_MAXNUM);
     }
     p.identity = to_acc_bits(init, t.vdt, p.acc);
     p.has_initial = r->has_initial ? 1 : 0;
@@ -292,7 +342,18 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
     }
     int64_t n_out = numel(rs);
     int64_t total = numel(in);
-    p.mean_div = n_out != 0 ? double(uint64_t(total) / uint64_t(n_out)) : 0.0;
+    if (t.fin == FZ_MEAN || t.fin == FZ_MEAN_SQRT) {
+        // static_cast<T-or-double>((size - ddof) / outputs), in size_t arithmetic
+        // (detail::mean / mean_division, core/xmath.hpp:1827-1850), then
+        // converted to the type of the division
+        const int64_t ddof = moment ? r->ddof : 0;
+        if (ddof < 0 || ddof > total) raise(XG_INVALID_ARGUMENT, "ddof out of range");
+        xg_scalar q{};
+        q.dtype = XG_UINT64;
+        q.v.u = n_out != 0 ? uint64_t(total - ddof) / uint64_t(n_out) : 0;
+        p.mdt = t.mdt;
+        p.div_bits = scalar_bits(scalar_of_bits(scalar_bits(q, t.div_dt), t.div_dt), t.mdt);
+    }
     if (n_out == 0) {
         p.n_out = 0;
         return p;
@@ -436,6 +497,7 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
         max_off = std::max(max_off, lmax);
         int64_t is = lv.stride[p.rank - 1];
         lv.mode = is == 0 ? LM_BCAST : (is == 1 && al) ? LM_CONTIG : LM_STRIDED;
+        if (lv.mode != LM_BCAST && streams(lv.stride, p.shape, p.rank, esz)) lv.mode |= LM_NT;
     }
     p.index64 = max_off >= (int64_t(1) << 31) ? 1 : 0;
 

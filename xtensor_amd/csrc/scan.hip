@@ -24,7 +24,7 @@ static bool scan_direct_ok(const ScanPlan& p) {
     if (p.prog.nops != 1 || p.prog.ops[0].op != MO_LOAD || p.prog.nleaves != 1) return false;
     const LeafView& l = p.leaves[0];
     const int esz = dtype_size(l.dtype);
-    if (l.dtype != in_dt[p.acc] || l.mode != LM_CONTIG || l.stride[p.rank - 1] != 1) return false;
+    if (l.dtype != in_dt[p.acc] || lm_base(l.mode) != LM_CONTIG || l.stride[p.rank - 1] != 1) return false;
     if (p.out_dtype == XG_BFLOAT16 || dtype_size(p.out_dtype) != esz || p.out_stride[p.rank - 1] != 1) return false;
     if (reinterpret_cast<uintptr_t>(l.ptr) % 16 || reinterpret_cast<uintptr_t>(p.out_ptr) % 16) return false;
     if (p.rank == 2 && ((l.stride[0] * esz) % 16 || (p.out_stride[0] * esz) % 16)) return false;

@@ -22,7 +22,7 @@ __device__ __forceinline__ int32_t coff(const int32_t (&st)[4], const uint32_t (
     return off;
 }
 
-template <int V>
+template <int V, bool NT = false>
 __device__ __forceinline__ void load_at(const char* p, int mode, int dt, int32_t is, int nvalid, Reg (&out)[V]) {
     const int esz = dtype_size(dt);
     if (mode == LM_BCAST) {
@@ -30,11 +30,11 @@ __device__ __forceinline__ void load_at(const char* p, int mode, int dt, int32_t
 #pragma unroll
         for (int v = 0; v < V; ++v) out[v] = x;
     } else if (mode == LM_CONTIG && nvalid == V) {
-        load_vec<V>(p, esz, dt, out);
+        load_vec<V, NT>(p, esz, dt, out);
     } else {
 #pragma unroll
         for (int v = 0; v < V; ++v)
-            out[v] = v < nvalid ? from_storage(load_raw(p + int64_t(v) * is * esz, esz), dt) : 0;
+            out[v] = v < nvalid ? from_storage(load_raw<NT>(p + int64_t(v) * is * esz, esz), dt) : 0;
     }
 }
 
@@ -46,6 +46,14 @@ __device__ __forceinline__ const char* leaf_ptr(const CompactPlan& p, int l) { r
 __device__ __forceinline__ const uint64_t* plan_consts(const CompactPlan& p) { return p.consts; }
 __device__ __forceinline__ const char* leaf_ptr(const RedPlan& p, int l) { return p.leaves[l].ptr; }
 __device__ __forceinline__ const uint64_t* plan_consts(const RedPlan& p) { return p.prog.consts; }
+// non-temporal loads for leaf l (a run-time flag of the plan: one compiled
+// program serves streaming and cache-resident operands)
+__device__ __forceinline__ bool leaf_nt(const EwPlan& p, int l) { return lm_nt(p.leaves[l].mode); }
+__device__ __forceinline__ bool leaf_nt(const CompactPlan&, int) { return false; }
+__device__ __forceinline__ bool leaf_nt(const RedPlan& p, int l) { return lm_nt(p.leaves[l].mode); }
+// non-temporal stores of the output
+__device__ __forceinline__ bool out_nt(const EwPlan& p) { return lm_nt(p.out_mode); }
+__device__ __forceinline__ bool out_nt(const CompactPlan&) { return false; }
 
 template <class S, int I, int V, int NR, int RANK, class P>
 __device__ __forceinline__ void run_spec(const P& plan, const uint32_t (&idx)[4], uint32_t j0, int nvalid,
@@ -54,10 +62,13 @@ __device__ __forceinline__ void run_spec(const P& plan, const uint32_t (&idx)[4]
         constexpr MicroOp m = S::ops[I];
         if constexpr (m.op == MO_LOAD) {
             constexpr int dt = S::leaf_dtype[m.aux];
-            constexpr int mode = S::leaf_mode[m.aux];
+            constexpr int mode = lm_base(S::leaf_mode[m.aux]);
             const int32_t off = coff<RANK, mode != LM_BCAST>(plan.cst[m.aux], idx, j0);
             const char* p = leaf_ptr(plan, m.aux) + int64_t(off) * dtype_size(dt);
-            load_at<V>(p, mode, dt, plan.cst[m.aux][RANK - 1], nvalid, R[m.dst]);
+            if (mode != LM_BCAST && leaf_nt(plan, m.aux))
+                load_at<V, true>(p, mode, dt, plan.cst[m.aux][RANK - 1], nvalid, R[m.dst]);
+            else
+                load_at<V>(p, mode, dt, plan.cst[m.aux][RANK - 1], nvalid, R[m.dst]);
         } else {
             Reg tmp[V];
 #pragma unroll

@@ -167,7 +167,7 @@ void emit_program(FILE* f, const Program& prog, const LeafView* leaves) {
     std::fprintf(f, "    static constexpr uint8_t leaf_dtype[%d] = {", nl);
     for (int l = 0; l < nl; ++l) std::fprintf(f, "%s%d", l ? ", " : "", l < prog.nleaves ? leaves[l].dtype : 0);
     std::fprintf(f, "};\n    static constexpr uint8_t leaf_mode[%d] = {", nl);
-    for (int l = 0; l < nl; ++l) std::fprintf(f, "%s%d", l ? ", " : "", l < prog.nleaves ? leaves[l].mode : 0);
+    for (int l = 0; l < nl; ++l) std::fprintf(f, "%s%d", l ? ", " : "", l < prog.nleaves ? lm_base(leaves[l].mode) : 0);
     std::fprintf(f, "};\n");
 }
 
@@ -195,7 +195,7 @@ int main(int argc, char** argv) {
         std::fprintf(f, "    static constexpr bool is_static = true;\n");
         std::fprintf(f, "    static constexpr int V = %d;\n", V);
         std::fprintf(f, "    static constexpr int out_dtype = %d;\n", p.out_dtype);
-        std::fprintf(f, "    static constexpr int out_mode = %d;\n", p.out_mode);
+        std::fprintf(f, "    static constexpr int out_mode = %d;\n", lm_base(p.out_mode));
         emit_program(f, p.prog, p.leaves);
         std::fprintf(f, "};\n\n");
         list += " X(" + sname + ")";

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "planner.hpp"
+#include "reduce.hpp"
 
 namespace xg {
 
@@ -21,6 +22,8 @@ void* workspace(size_t bytes, hipStream_t stream);
 // Plan out = e with the compiled-kernel vector width, falling back to the
 // generic interpreter's width when no compiled program matches.
 EwPlan plan_assign_best(const xg_expr* e, const xg_tensor* out, int* V);
+// Plan out = reduce(e) the same way.
+RedPlan plan_reduce_best(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, int* V);
 
 // Enqueue many elementwise plans: plans of one compiled program and rank go
 // out as one fused launch (tables uploaded through the stream's staging

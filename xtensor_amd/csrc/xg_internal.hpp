@@ -79,8 +79,17 @@ constexpr int kMaxRank = XG_MAX_RANK;
 enum LeafMode : uint8_t {
     LM_CONTIG = 0,  // inner stride 1 and vector-aligned: wide loads
     LM_BCAST = 1,   // inner stride 0: one load per chunk
-    LM_STRIDED = 2  // anything else: one load per element
+    LM_STRIDED = 2, // anything else: one load per element
+    // flag on CONTIG / STRIDED: a large leaf whose every element is read by
+    // exactly one work item (no broadcast reuse) is loaded non-temporally,
+    // so the stream does not evict cached lines (dirty ones included) that
+    // it will never hit
+    LM_NT = 8
 };
+XG_HD constexpr int lm_base(int mode) { return mode & 7; }
+XG_HD constexpr bool lm_nt(int mode) { return (mode & LM_NT) != 0; }
+// leaves spanning at least this many bytes stream non-temporally
+constexpr int64_t kNtMinBytes = int64_t(64) << 20;
 
 // Division by a run-time constant with a precomputed multiplier
 // (Granlund & Montgomery, "Division by invariant integers using
