@@ -457,6 +457,21 @@ def main():
         extra["generic_interpreter_GBps"] = round(nbytes / (dkms * 1e-3) / 1e9, 1)
     finally:
         A.lib().xg_debug_force_dynamic(0)
+    # a program outside the ahead-of-time table (the same workload with a
+    # float literal, so every operation is float32): compiled at run time on
+    # first use, then launched like a table program
+    t0 = time.perf_counter()
+    f32lit = xt.scalar(1e-5, "float32")
+    jplans = [xt.plan(xt.view(y, *view), xt.clip((xt.view(x, *view) - params[0]) / xt.sqrt(params[1] + f32lit)
+                                                  * params[2] + params[3], -3, 3)) for x, y in sets]
+    jplans[0].run()
+    torch.cuda.synchronize()
+    compile_s = time.perf_counter() - t0
+    jkernel = xt.last_kernel()
+    _, jkms = time_plans(torch, jplans, max(10, args.steps // 2), 3)
+    extra["runtime_compiled_f32_literal"] = {"kernel": jkernel, "GBps": round(nbytes / (jkms * 1e-3) / 1e9, 1),
+                                             "first_call_s": round(compile_s, 2)}
+    del jplans
     # ceilings of the access pattern on this GPU: a plain copy through the
     # same views, and a contiguous copy of the same number of bytes (both
     # rotating over the three sets, through the compiled copy program)

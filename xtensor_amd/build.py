@@ -110,7 +110,8 @@ def build(force: bool = False, jobs: int | None = None) -> str:
             list(ex.map(_run, todo))
     if todo or not os.path.exists(LIB) or _newest(objs) > os.path.getmtime(LIB):
         tmp = LIB + ".tmp"
-        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs +
+             ["-L/opt/rocm/lib", "-lhiprtc", "-ldl", "-Wl,-rpath,/opt/rocm/lib"])
         os.replace(tmp, LIB)
 
     # 3. the C++ front end's test program (host code against libxg.so)

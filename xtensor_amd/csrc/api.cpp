@@ -15,6 +15,7 @@
 #include <map>
 
 #include "ew.hpp"
+#include "jit.hpp"
 #include "planner.hpp"
 #include "reduce.hpp"
 #include "scan.hpp"
@@ -105,7 +106,7 @@ using namespace xg;
 EwPlan xg::plan_assign_best(const xg_expr* e, const xg_tensor* out, int* V) {
     *V = choose_vector_width(e, out);
     EwPlan p = plan_assign(e, out, *V);
-    if (*V != 4 && !ew_find_spec(p, *V)) {
+    if (*V != 4 && !ew_find_spec(p, *V) && !ew_jit_ok(p, *V)) {
         *V = 4;
         p = plan_assign(e, out, 4);
     }
@@ -118,7 +119,7 @@ RedPlan xg::plan_reduce_best(const xg_expr* e, const xg_reduce_desc* r, const xg
     const int cus = device_cus();
     *V = choose_vector_width(e, nullptr);
     RedPlan p = plan_reduce(e, r, out, *V, cus, nullptr, nullptr);
-    if (*V != 4 && !find_rspec(p, *V)) {
+    if (*V != 4 && !find_rspec(p, *V) && !reduce_jit_ok(p, *V)) {
         *V = 4;
         p = plan_reduce(e, r, out, *V, cus, nullptr, nullptr);
     }
@@ -494,6 +495,23 @@ void xg_debug_units_per_thread(int upt) { ew_set_units_per_thread(upt); }
 // non-temporal stores of large outputs (both on by default), for plans made
 // after the call.
 void xg_debug_nontemporal(int loads, int stores) { set_nontemporal(loads, stores); }
+
+// Run-time compiled specialisations: 0 off, 1 for launches of at least
+// XG_JIT_MIN_ELEMENTS elements (default), 2 for every compact plan.
+void xg_debug_jit(int mode) { jit_set_mode(mode); }
+
+// Counters of the run-time compiler: compiles, disk-cache hits, in-process
+// hits, failures (out[0..3]); returns the last failure message.
+const char* xg_debug_jit_stats(int64_t* out) {
+    JitStats s = jit_stats();
+    if (out) {
+        out[0] = s.compiles;
+        out[1] = s.disk_hits;
+        out[2] = s.memory_hits;
+        out[3] = s.failures;
+    }
+    return jit_last_error();
+}
 
 // Name of the compiled specialisation an assignment would use, "" if none.
 xg_status xg_debug_assign_kernel(const xg_expr* e, const xg_tensor* out, char* name, int32_t cap) {

@@ -17,6 +17,7 @@
 #include "spec_eval.hpp"
 #include "ew.hpp"
 #include "ew_kernels.hpp"
+#include "jit.hpp"
 #include "planner.hpp"
 
 namespace xg {
@@ -126,6 +127,10 @@ int g_units_per_thread = 0;  // 0: choose by size; 1 or 4: forced (measurement a
 }  // namespace
 
 void ew_set_force_dynamic(int on) { g_force_dynamic = on; }
+
+bool ew_jit_ok(const EwPlan& p, int V) {
+    return !g_force_dynamic && p.compact && jit_wanted(p.units * uint64_t(V)) && jit_ew(p, V) != nullptr;
+}
 void ew_set_units_per_thread(int upt) { g_units_per_thread = (upt == 1 || upt == 4) ? upt : 0; }
 
 const char* ew_find_spec(const EwPlan& p, int V) {
@@ -209,6 +214,11 @@ hipError_t launch_ew(const EwPlan& p, int V, int cus, hipStream_t stream, const 
                 return hipGetLastError();
             }
         }
+    }
+    if (ew_jit_ok(p, V)) {  // a run-time compiled specialisation
+        const JitKernel* k = jit_ew(p, V);
+        if (used) *used = k->name.c_str();
+        return jit_launch(k, grid.x, 256, &p, sizeof(EwPlan), stream);
     }
     if (V != 4) return hipErrorInvalidValue;  // the generic interpreter is built for V = 4
     if (used) *used = "dynamic";

@@ -7,15 +7,20 @@
 
 namespace xg {
 
+#if !defined(__HIPCC_RTC__)  // host API (absent from run-time compiled sources)
 // Launch out = program(leaves) for a plan built with vector width V.
 // `used` (optional) receives the name of the kernel that ran.
 hipError_t launch_ew(const EwPlan& p, int V, int cus, hipStream_t stream, const char** used);
+#endif
 
 // Name of the compiled specialisation that matches the plan, or nullptr.
 const char* ew_find_spec(const EwPlan& p, int V);
 
 // Route every launch through the dynamic interpreter (testing aid).
 void ew_set_force_dynamic(int on);
+// True when a plan without an ahead-of-time program runs a run-time compiled
+// one (compiling it on first use).
+bool ew_jit_ok(const EwPlan& p, int V);
 // Force one (1) or four (4) work units per thread in compiled kernels, or
 // choose by size (0) (measurement aid).
 void ew_set_units_per_thread(int upt);
@@ -43,7 +48,9 @@ CompactPlan make_compact(const EwPlan& p);
 // Evaluate n plans of compiled program `spec_index` at normalised rank
 // `rank` in one launch.  d_plans / d_prefix live in device memory;
 // d_prefix[i] is the first work unit of plan i, d_prefix[n] the total.
+#if !defined(__HIPCC_RTC__)
 hipError_t launch_ew_multi(int spec_index, int rank, const CompactPlan* d_plans, const uint64_t* d_prefix, uint32_t n,
                            uint64_t total_units, hipStream_t stream);
+#endif
 
 }  // namespace xg

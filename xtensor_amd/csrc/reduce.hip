@@ -16,9 +16,11 @@
 // exact two's-complement arithmetic of the value type.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
+#include "jit.hpp"
 #include "reduce_kernels.hpp"
 
 
@@ -95,13 +97,37 @@ static hipError_t launch_combine(const RedPlan& p, hipStream_t stream) {
     return hipGetLastError();
 }
 
+namespace {
+
+uint64_t plan_elements(const RedPlan& p) {
+    uint64_t n = 1;
+    for (int k = 0; k < p.rank; ++k) n *= uint64_t(std::max<int64_t>(p.shape[k], 0));
+    return n;
+}
+
+// the kernel template a plan runs: a fast one for compact plans
+int jit_kind(const RedPlan& p) {
+    if (p.compact) return p.layout == RL_ROWS ? (p.rank == 1 ? JR_ROWS_FAST1 : JR_ROWS_FAST2) : JR_COLS_FAST;
+    return p.layout == RL_ROWS ? JR_ROWS : JR_COLS;
+}
+
+}  // namespace
+
+bool reduce_jit_ok(const RedPlan& p, int V) {
+    return !p.index64 && p.n_out > 0 && jit_wanted(plan_elements(p)) && jit_reduce(p, V, jit_kind(p)) != nullptr;
+}
+
 hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** used) {
     if (p.n_out == 0) return hipSuccess;
     RedKernels ks{};
+    const JitKernel* jk = nullptr;
     const RSpecEntry* sp = p.index64 ? nullptr : find_rspec(p, V);
     if (sp) {
         ks = sp->k;
         if (used) *used = sp->name;
+    } else if (reduce_jit_ok(p, V)) {  // a run-time compiled specialisation
+        jk = jit_reduce(p, V, jit_kind(p));
+        if (used) *used = jk->name.c_str();
     } else {
         if (V != 4) return hipErrorInvalidValue;  // dynamic kernels are built for V = 4
         if (!dyn_reduce_float(p.acc, p.op, &ks) && !dyn_reduce_int32(p.acc, p.op, &ks) &&
@@ -109,46 +135,49 @@ hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** use
             return hipErrorInvalidValue;
         if (used) *used = "dynamic";
     }
-    dim3 block(256);
-    if (sp && p.compact && (p.layout == RL_ROWS ? (p.rank == 1 ? ks.rows_fast1 : ks.rows_fast2) : ks.cols_fast)) {
+    auto go = [&](KFn f, uint32_t grid, uint32_t block, const RedPlan& q) -> hipError_t {
+        if (jk) return jit_launch(jk, grid, block, &q, sizeof(RedPlan), stream);
+        if (!f) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(f, dim3(grid), dim3(block), 0, stream, q);
+        return hipGetLastError();
+    };
+    const bool fast = p.compact &&
+                      (jk || (sp && (p.layout == RL_ROWS ? (p.rank == 1 ? ks.rows_fast1 : ks.rows_fast2) : ks.cols_fast)));
+    if (fast) {
         // compiled fast kernels with their own split of the work
         RedPlan q = p;
         q.group = p.f_group;
         q.splits = p.f_splits;
         q.per_split = p.f_per_split;
         q.div_splits = make_fastdiv(uint32_t(p.f_splits));
+        hipError_t err;
         if (q.layout == RL_ROWS) {
             const uint64_t items = uint64_t(q.n_out) * uint64_t(q.splits);
             const uint64_t per_block = 4 * (64 / uint64_t(q.group));
             const uint64_t blocks = (items + per_block - 1) / per_block;
             if (blocks >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
-            hipLaunchKernelGGL(q.rank == 1 ? ks.rows_fast1 : ks.rows_fast2, dim3(uint32_t(blocks)), block, 0, stream, q);
+            err = go(q.rank == 1 ? ks.rows_fast1 : ks.rows_fast2, uint32_t(blocks), 256, q);
         } else {
             const uint64_t tiles = (q.nchunks_f + 63) / 64;
             const uint64_t blocks = tiles * uint64_t(q.splits);
             if (blocks >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
-            hipLaunchKernelGGL(ks.cols_fast, dim3(uint32_t(blocks)), dim3(64 * kColsWaves), 0, stream, q);
+            err = go(ks.cols_fast, uint32_t(blocks), 64 * kColsWaves, q);
         }
-        hipError_t err = hipGetLastError();
         if (err != hipSuccess || q.splits == 1) return err;
         return launch_combine(q, stream);
     }
+    hipError_t err;
     if (p.layout == RL_ROWS) {
         uint64_t items = uint64_t(p.n_out) * p.splits;
         uint64_t per_block = 256 / p.group;
         uint64_t blocks = (items + per_block - 1) / per_block;
         blocks = std::min<uint64_t>(std::max<uint64_t>(blocks, 1), 1u << 20);
-        KFn k = p.index64 ? ks.rows64 : ks.rows32;
-        if (!k) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k, dim3(uint32_t(blocks)), block, 0, stream, p);
+        err = go(p.index64 ? ks.rows64 : ks.rows32, uint32_t(blocks), 256, p);
     } else {
         uint64_t items = uint64_t(p.n_out / p.lf) * p.col_tiles * p.splits;
         uint64_t blocks = std::min<uint64_t>(std::max<uint64_t>(items, 1), 1u << 20);
-        KFn k = p.index64 ? ks.cols64 : ks.cols32;
-        if (!k) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k, dim3(uint32_t(blocks)), block, 0, stream, p);
+        err = go(p.index64 ? ks.cols64 : ks.cols32, uint32_t(blocks), 256, p);
     }
-    hipError_t err = hipGetLastError();
     if (err != hipSuccess || p.splits == 1) return err;
     return launch_combine(p, stream);
 }

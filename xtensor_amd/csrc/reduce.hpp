@@ -99,12 +99,17 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
 // Result dtype / shape only.
 void reduce_result(const xg_expr* e, const xg_reduce_desc* r, int* dtype, Shape* shape);
 
+#if !defined(__HIPCC_RTC__)  // host API (absent from run-time compiled sources)
 hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** used);
+#endif
 
 size_t reduce_workspace_bytes(const RedPlan& p);
 
 struct RSpecEntry;
 // compiled specialisation matching the plan, or nullptr
 const RSpecEntry* find_rspec(const RedPlan& p, int V);
+// True when a plan without an ahead-of-time program runs a run-time compiled
+// one (compiling it on first use).
+bool reduce_jit_ok(const RedPlan& p, int V);
 
 }  // namespace xg

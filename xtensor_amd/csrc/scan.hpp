@@ -44,6 +44,8 @@ void scan_result(const xg_expr* e, const xg_scan_desc* s, int* dtype, Shape* sha
 // Plan out = scan(e) along an axis (not the flattened form).
 ScanPlan plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, int V);
 
+#if !defined(__HIPCC_RTC__)
 hipError_t launch_scan(const ScanPlan& p, int V, hipStream_t stream, const char** used);
+#endif
 
 }  // namespace xg

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "builder.hpp"
+#include "spec_emit.hpp"
 #include "planner.hpp"
 #include "reduce.hpp"
 
@@ -152,25 +153,6 @@ const std::vector<RSpecDef>& rspec_defs() {
     return defs;
 }
 
-void emit_program(FILE* f, const Program& prog, const LeafView* leaves) {
-    std::fprintf(f, "    static constexpr int nops = %d;\n", prog.nops);
-    std::fprintf(f, "    static constexpr int nregs = %d;\n", prog.nregs);
-    std::fprintf(f, "    static constexpr int nleaves = %d;\n", prog.nleaves);
-    std::fprintf(f, "    static constexpr int result_reg = %d;\n", prog.result_reg);
-    std::fprintf(f, "    static constexpr xg::MicroOp ops[%d] = {", prog.nops);
-    for (int i = 0; i < prog.nops; ++i) {
-        const MicroOp& m = prog.ops[i];
-        std::fprintf(f, "%s{%d, %d, %d, %d, %d, %d, %d}", i ? ", " : "", m.op, m.t, m.dst, m.a, m.b, m.c, m.aux);
-    }
-    std::fprintf(f, "};\n");
-    int nl = prog.nleaves > 0 ? prog.nleaves : 1;
-    std::fprintf(f, "    static constexpr uint8_t leaf_dtype[%d] = {", nl);
-    for (int l = 0; l < nl; ++l) std::fprintf(f, "%s%d", l ? ", " : "", l < prog.nleaves ? leaves[l].dtype : 0);
-    std::fprintf(f, "};\n    static constexpr uint8_t leaf_mode[%d] = {", nl);
-    for (int l = 0; l < nl; ++l) std::fprintf(f, "%s%d", l ? ", " : "", l < prog.nleaves ? lm_base(leaves[l].mode) : 0);
-    std::fprintf(f, "};\n");
-}
-
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -191,13 +173,7 @@ int main(int argc, char** argv) {
             return 2;
         }
         std::string sname = std::string("Spec_") + d.name;
-        std::fprintf(f, "struct %s {\n", sname.c_str());
-        std::fprintf(f, "    static constexpr bool is_static = true;\n");
-        std::fprintf(f, "    static constexpr int V = %d;\n", V);
-        std::fprintf(f, "    static constexpr int out_dtype = %d;\n", p.out_dtype);
-        std::fprintf(f, "    static constexpr int out_mode = %d;\n", lm_base(p.out_mode));
-        emit_program(f, p.prog, p.leaves);
-        std::fprintf(f, "};\n\n");
+        std::fprintf(f, "%s\n", emit_ew_spec(sname, p, V).c_str());
         list += " X(" + sname + ")";
     }
     std::fprintf(f, "%s\n\n", list.c_str());
@@ -224,13 +200,7 @@ int main(int argc, char** argv) {
             return 2;
         }
         std::string sname = std::string("RSpec_") + d.name;
-        std::fprintf(f, "struct %s {\n", sname.c_str());
-        std::fprintf(f, "    static constexpr bool is_static = true;\n");
-        std::fprintf(f, "    static constexpr int V = %d;\n", V);
-        std::fprintf(f, "    static constexpr int acc = %d;\n", p.acc);
-        std::fprintf(f, "    static constexpr int op = %d;\n", p.op);
-        emit_program(f, p.prog, p.leaves);
-        std::fprintf(f, "};\n\n");
+        std::fprintf(f, "%s\n", emit_red_spec(sname, p, V).c_str());
         rlist += " X(" + sname + ")";
     }
     std::fprintf(f, "%s\n", rlist.c_str());
