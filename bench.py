@@ -378,9 +378,14 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--profile-batch", action="store_true",
                     help="profiling pass for rocprofv3: the uneven softmax batch, one launch and 3 graph replays")
+    ap.add_argument("--profile-reduce", action="store_true",
+                    help="profiling pass for rocprofv3: a 256 MiB float32 copy (counter calibration), then "
+                         "sum axis 0 / axis 1, amax over all, cumsum axis 1 of f32 [8192, 4096], 3 launches "
+                         "each over three input copies")
     ap.add_argument("--profile", action="store_true",
-                    help="profiling pass for rocprofv3: a 256 MiB float32 copy (counter calibration) "
-                         "and the workload kernel, 3 launches each, nothing else")
+                    help="profiling pass for rocprofv3: a 256 MiB float32 copy (counter calibration, 3 "
+                         "launches) and the workload kernel as in the timed loop (6 + 12 launches rotating "
+                         "over the three sets), nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (single-GPU build)", file=sys.stderr)
@@ -402,6 +407,29 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"profile_pass": "uneven_batch_softmax_f32", "elements": elems,
                           "algorithmic_bytes_each": 8 * elems, "tuning": b.tuning, "tasks": b.info()["tasks"]}))
+        return
+    if args.profile_reduce:
+        src = xt.empty((64 << 20,), "float32")
+        src.storage.zero_()
+        dst = xt.empty((64 << 20,), "float32")
+        cp = xt.plan(dst, src)
+        for _ in range(3):
+            cp.run()
+        X = np.random.default_rng(2).normal(1.0, 1.0, (8192, 4096)).astype(np.float32)
+        dX = [xt.from_numpy(X) for _ in range(3)]
+        names = {}
+        for name, make, oshape in (("sum_axis0", lambda d: xt.sum(d, [0]), (4096,)),
+                                   ("sum_axis1", lambda d: xt.sum(d, [1]), (8192,)),
+                                   ("amax_all", lambda d: xt.amax(d), ()),
+                                   ("cumsum_axis1", lambda d: xt.Scan("cumsum", d, 1), (8192, 4096))):
+            outs = [xt.empty(oshape, "float32") for _ in dX]
+            plans = [xt.plan(o, make(d)) for o, d in zip(outs, dX)]
+            for p in plans:
+                p.run()
+            names[name] = xt.last_kernel()
+            torch.cuda.synchronize()
+        print(json.dumps({"profile_pass": "reductions_f32_8192x4096", "copy_bytes_each": 2 * (256 << 20),
+                          "input_bytes": X.nbytes, "kernels": names}))
         return
     x_h, mean_h, var_h, gamma_h, beta_h = make_inputs(np)
     params = [xt.from_numpy(v) for v in (mean_h, var_h, gamma_h, beta_h)]
@@ -426,8 +454,8 @@ def main():
         cp = xt.plan(dst, src)
         for _ in range(3):
             cp.run()
-        for i in range(3):
-            plans[i % len(plans)].run()
+        # the timed loop's steady state: launches rotating over the three sets
+        time_plans(torch, plans, 12, 6)
         torch.cuda.synchronize()
         print(json.dumps({"profile_pass": True, "copy_bytes_each": 2 * (256 << 20),
                           "workload_algorithmic_bytes": algorithmic_bytes(), "kernel": kernel_name}))
