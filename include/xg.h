@@ -286,6 +286,22 @@ xg_status xg_scan_info(const xg_expr* e, const xg_scan_desc* s, int32_t* dtype, 
 /* out = cumsum/cumprod(e, axis). */
 xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, xg_stream_t stream);
 
+/* ---- .npy files to and from device memory (io/xnpy.hpp) ------------------
+ * Header of an .npy file: dtype, shape and order (xt::detail::load_npy_file's
+ * header parse, io/xnpy.hpp:236-393, 646-673).  A type string without a
+ * dtype of this host (big-endian, complex) is an error, as load_npy<T>'s
+ * "Cast error" (io/xnpy.hpp:588-596). */
+xg_status xg_npy_info(const char* path, int32_t* dtype, int32_t* ndim, int64_t* shape, int32_t* fortran_order);
+/* Read the file's data into out: xt::load_npy<T>(path) (io/xnpy.hpp:775-800).
+ * out must have the file's dtype and shape and may have any strides; a
+ * Fortran-order file fills it in column-major element order.  Returns when
+ * the data is on the device. */
+xg_status xg_npy_load(const char* path, const xg_tensor* in_out, xg_stream_t stream);
+/* Write a device tensor (any strides) as a C-order file with xtensor's
+ * header layout: xt::dump_npy(path, e) (io/xnpy.hpp:396-473, 709-755).
+ * Returns when the file is written. */
+xg_status xg_npy_dump(const char* path, const xg_tensor* in, xg_stream_t stream);
+
 /* ---- prepared plans --------------------------------------------------------
  * Plan once on the host, launch many times: a plan captures the normalised
  * iteration space, the lowered program and the kernel choice for the given

@@ -8,6 +8,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
+#include <iterator>
+#include <unistd.h>
 #include <random>
 #include <string>
 #include <vector>
@@ -102,6 +105,28 @@ static void cpu_tests() {
 static double ulp_rel(double got, double ref) {
     if (got == ref) return 0.0;
     return std::fabs(got - ref) / std::fabs(std::nextafter(ref, 2 * ref + 1) - ref);
+}
+
+// .npy files (io/xnpy.hpp) against the reference's fixture files in `dir`
+static void npy_tests(const std::string& dir) {
+    auto ia = xt::load_npy<int32_t>(dir + "/int.le.npy");  // test_xnpy.cpp:57, 80-81
+    EXPECT(ia.to_host() == std::vector<int32_t>({3, 4, 5, 6, 7}));
+    bool threw = false;
+    try {
+        (void)xt::load_npy<double>(dir + "/int.le.npy");
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    EXPECT(threw);
+    auto u = xt::xdevice_array<uint64_t>::from_host({12, 14, 16, 18, 1234321}, {5});  // test_xnpy.cpp:144
+    const std::string tmp = "/tmp/xg_test_dump_" + std::to_string(::getpid()) + ".npy";
+    xt::dump_npy(tmp, u);
+    auto slurp = [](const std::string& p) {
+        std::ifstream f(p, std::ios::binary);
+        return std::string((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    };
+    EXPECT(!slurp(tmp).empty() && slurp(tmp) == slurp(dir + "/unsignedlong.le.npy"));
+    std::remove(tmp.c_str());
 }
 
 static void gpu_tests() {
@@ -205,6 +230,7 @@ int main(int argc, char** argv) {
     try {
         cpu_tests();
         if (mode == "gpu") gpu_tests();
+        if (mode == "gpu" && argc > 2) npy_tests(argv[2]);
     } catch (const std::exception& e) {
         std::printf("FAILED: exception %s\n", e.what());
         return 1;

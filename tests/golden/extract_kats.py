@@ -30,6 +30,7 @@ FILES = [
     "test_xbroadcast.cpp",
     "test_strided_assign.cpp",
     "test_extended_xmath_reducers.cpp",
+    "test_xnpy.cpp",
 ]
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kat_arrays.json")
 

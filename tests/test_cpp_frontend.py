@@ -30,5 +30,6 @@ def test_cpp_frontend_cpu():
 
 @pytest.mark.gpu
 def test_cpp_frontend_gpu():
-    r = subprocess.run([_exe(), "gpu"], capture_output=True, text=True, timeout=120)
+    npy_dir = os.path.join(ROOT, "tests", "golden", "xnpy")
+    r = subprocess.run([_exe(), "gpu", npy_dir], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout + r.stderr

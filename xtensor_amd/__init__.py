@@ -10,9 +10,10 @@ ABI of `include/xg.h`; there is no CPU fallback.
 from .core import (  # noqa: F401
     BroadcastError, DeviceArray, DeviceError, Expr, Func, Reducer, Scalar, Scan, assign_checked,
     abs, acos, all_, amax, amin, asarray, asin, assign, atan, atan2, broadcast, cast, cbrt, ceil, clip,
-    cos, cosh, cumprod, cumsum, dtype_code, dtype_name, empty, equal, erf, eval, exp, exp2, expm1,
+    cos, cosh, cumprod, cumsum, dtype_code, dtype_name, dump_npy, empty, equal, erf, eval, exp, exp2, expm1,
     expr_info, fabs, floor, fmod, from_numpy, greater, hypot, last_kernel, less, log, log1p, log2,
-    log10, logical_and, logical_not, logical_or, maximum, mean, minimum, minmax, newaxis, noalias,
+    load_npy, log10, logical_and, logical_not, logical_or, maximum, mean, minimum, minmax, newaxis, noalias,
+    npy_info,
     norm_l0, norm_l1, norm_l2, norm_linf, norm_sq, not_equal, numpy_dtype, plan, Plan, pow, prod, range_,
     reduce_info, reshape_view, round, scalar, serialize, sin, sinh, sqrt, stddev, strided_view, sum, tan,
     tanh, transpose, trunc, variance, view, where, zeros,

@@ -150,6 +150,9 @@ EXPORTS = {
     "xg_reduce": (C.c_int, [_P(Expr), _P(ReduceDesc), _P(Tensor), C.c_void_p]),
     "xg_scan_info": (C.c_int, [_P(Expr), _P(ScanDesc), _P(C.c_int32), _P(C.c_int32), _P(C.c_int64)]),
     "xg_scan": (C.c_int, [_P(Expr), _P(ScanDesc), _P(Tensor), C.c_void_p]),
+    "xg_npy_info": (C.c_int, [C.c_char_p, _P(C.c_int32), _P(C.c_int32), _P(C.c_int64), _P(C.c_int32)]),
+    "xg_npy_load": (C.c_int, [C.c_char_p, _P(Tensor), C.c_void_p]),
+    "xg_npy_dump": (C.c_int, [C.c_char_p, _P(Tensor), C.c_void_p]),
     "xg_plan_assign": (C.c_int, [_P(Expr), _P(Tensor), _P(C.c_void_p)]),
     "xg_plan_reduce": (C.c_int, [_P(Expr), _P(ReduceDesc), _P(Tensor), _P(C.c_void_p)]),
     "xg_plan_scan": (C.c_int, [_P(Expr), _P(ScanDesc), _P(Tensor), _P(C.c_void_p)]),
@@ -197,6 +200,10 @@ def lib():
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
                 "(the evaluator has no CPU fallback)")
+        # torch (the allocator and stream provider) brings its own HIP runtime;
+        # it must be loaded before this library's, or the library's runtime
+        # finds no device once torch has initialised the GPU (measured on MI355X)
+        import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in {**EXPORTS, **DEBUG_EXPORTS}.items():
             f = getattr(L, name, None)

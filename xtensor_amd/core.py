@@ -29,6 +29,7 @@ from __future__ import annotations
 import builtins
 import ctypes as C
 import numbers
+import os
 from typing import Any, Callable, Iterable, Sequence
 
 import numpy as np
@@ -802,6 +803,44 @@ def eval(e) -> DeviceArray:  # noqa: A001
     dt, shp = expr_info(e)
     out = empty(shp, dt)
     return assign(out, e)
+
+
+# ----------------------------------------------------------------------------
+# .npy files (io/xnpy.hpp), straight to and from device memory
+def npy_info(path):
+    """(dtype name, shape, fortran_order) of an .npy file's header."""
+    dt, nd, fo = C.c_int32(), C.c_int32(), C.c_int32()
+    shp = (C.c_int64 * A.MAX_RANK)()
+    _check(A.lib().xg_npy_info(os.fsencode(path), C.byref(dt), C.byref(nd), shp, C.byref(fo)))
+    return dtype_name(dt.value), tuple(shp[k] for k in range(nd.value)), bool(fo.value)
+
+
+def load_npy(path, dtype=None) -> DeviceArray:
+    """xt::load_npy<T>(path) (io/xnpy.hpp:775-800): the file's data in a new
+    device array; a Fortran-order file gives a column-major array.  `dtype`,
+    if given, must be the file's ("Cast error: formats not matching")."""
+    dt, shape, fortran = npy_info(path)
+    if dtype is not None and dtype_code(dtype) != dtype_code(dt):
+        raise RuntimeError(f"Cast error: formats not matching: file holds {dt}, asked for {dtype_name(dtype_code(dtype))}")
+    a = empty(shape, dt)
+    if fortran:
+        st, s = [], 1
+        for n in shape:
+            st.append(s)
+            s *= max(int(n), 1)
+        a = DeviceArray(a.storage, a.dtype, shape, tuple(st), 0)
+    t = a.to_abi()
+    _check(A.lib().xg_npy_load(os.fsencode(path), C.byref(t), _stream()))
+    return a
+
+
+def dump_npy(path, e) -> None:
+    """xt::dump_npy(path, e) (io/xnpy.hpp:709-755): evaluates e and writes a
+    C-order .npy file with the reference's header layout."""
+    a = e if isinstance(e, DeviceArray) else eval(e)
+    _require_device([a])
+    t = a.to_abi()
+    _check(A.lib().xg_npy_dump(os.fsencode(path), C.byref(t), _stream()))
 
 
 # ----------------------------------------------------------------------------

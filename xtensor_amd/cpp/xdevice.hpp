@@ -20,6 +20,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <initializer_list>
@@ -607,5 +608,37 @@ template <class T, class E> xdevice_array<T> scan(int op, const E& e, int axis) 
 }
 template <class T, class E> xdevice_array<T> cumsum(const E& e, int axis) { return scan<T>(XG_SCAN_CUMSUM, e, axis); }
 template <class T, class E> xdevice_array<T> cumprod(const E& e, int axis) { return scan<T>(XG_SCAN_CUMPROD, e, axis); }
+
+// ---------------------------------------------------------------- .npy files
+// xt::load_npy<T>(path) (io/xnpy.hpp:775-800): the file's data in a new device
+// array; a Fortran-order file gives column-major strides, as the reference
+// adapts its buffer.  The file's type string must be T's ("Cast error").
+template <class T> xdevice_array<T> load_npy(const std::string& path) {
+    int32_t dt = 0, nd = 0, fortran = 0;
+    int64_t shp[XG_MAX_RANK];
+    check(xg_npy_info(path.c_str(), &dt, &nd, shp, &fortran));
+    if (dt != dtype_of<T>::value) throw std::runtime_error("Cast error: formats not matching");
+    shape_type sh(shp, shp + nd);
+    xdevice_array<T> a(sh);
+    if (fortran) {
+        shape_type st(sh.size(), 1);
+        for (size_t k = 1; k < sh.size(); ++k) st[k] = st[k - 1] * std::max<int64_t>(sh[k - 1], 1);
+        a = a.reshaped_view(sh, st, 0);
+    }
+    xg_tensor t = a.tensor();
+    check(xg_npy_load(path.c_str(), &t, current_stream()));
+    return a;
+}
+
+// xt::dump_npy(path, e) (io/xnpy.hpp:709-755): writes a C-order file with
+// the reference's header layout; an expression is evaluated first (as T).
+template <class T> void dump_npy(const std::string& path, const xdevice_array<T>& a) {
+    xg_tensor t = a.tensor();
+    check(xg_npy_dump(path.c_str(), &t, current_stream()));
+}
+template <class T, class E, std::enable_if_t<is_dev_expr<E>::value, int> = 0>
+void dump_npy(const std::string& path, const E& e) {
+    dump_npy(path, eval<T>(e));
+}
 
 }  // namespace xtensor_amd
