@@ -172,7 +172,13 @@ typedef enum xg_op {
     XG_OP_FLOOR = 71,
     XG_OP_CEIL = 72,
     XG_OP_TRUNC = 73,
-    XG_OP_ROUND = 74
+    XG_OP_ROUND = 74,
+    /* gather: the element of leaf `index` at element offset args[0] (an
+     * integer expression) from the leaf's own offset, i.e. data[offset + v]:
+     * xt::index_view / xt::filter read through a list of positions
+     * (views/xindex_view.hpp:786-826).  The leaf does not take part in the
+     * broadcast; offsets outside its extent read as 0. */
+    XG_OP_GATHER = 80
 } xg_op;
 
 /* One node of an expression tree.  Nodes are listed so that every argument
@@ -285,6 +291,15 @@ xg_status xg_scan_info(const xg_expr* e, const xg_scan_desc* s, int32_t* dtype, 
 
 /* out = cumsum/cumprod(e, axis). */
 xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, xg_stream_t stream);
+
+/* Assignment into an index view: target's element at offset offsets[i] (from
+ * the target's own offset) = values[i], for the 1-d integer expression
+ * `offsets` and `values` broadcast to its length.  Replaces the assignment of
+ * xt::index_view(e, indices) = values (views/xindex_view.hpp:786-797).  Both
+ * are evaluated before any store; offsets outside the target are skipped;
+ * repeated offsets leave one of their values. */
+xg_status xg_assign_indexed(const xg_expr* values, const xg_expr* offsets, const xg_tensor* target,
+                            xg_stream_t stream);
 
 /* ---- .npy files to and from device memory (io/xnpy.hpp) ------------------
  * Header of an .npy file: dtype, shape and order (xt::detail::load_npy_file's

@@ -474,6 +474,8 @@ void check_expr(const xg_expr* e) {
             if (n.args[k] < 0 || n.args[k] >= i) fail(XG_INVALID_ARGUMENT, "node argument out of order");
         if (n.op == XG_OP_LEAF && (n.index < 0 || n.index >= e->n_leaves)) fail(XG_INVALID_ARGUMENT, "bad leaf");
         if (n.op == XG_OP_CONST && (n.index < 0 || n.index >= e->n_consts)) fail(XG_INVALID_ARGUMENT, "bad const");
+        if (n.op == XG_OP_GATHER && (n.nargs != 1 || n.index < 0 || n.index >= e->n_leaves))
+            fail(XG_INVALID_ARGUMENT, "bad gather");
     }
     for (int l = 0; l < e->n_leaves; ++l)
         if (e->leaves[l].ndim < 0 || e->leaves[l].ndim > XG_MAX_RANK) fail(XG_INVALID_ARGUMENT, "bad leaf rank");
@@ -510,6 +512,9 @@ Tree analyse(const xg_expr* e) {
         } else if (n.op == XG_OP_CONST) {
             t.node_dt[i] = e->consts[n.index].dtype;
             t.node_rank[i] = 0;
+        } else if (n.op == XG_OP_GATHER) {  // an element of the leaf: its value type, the offsets' rank
+            t.node_dt[i] = value_dt_of_storage(e->leaves[n.index].dtype);
+            t.node_rank[i] = t.node_rank[n.args[0]];
         } else {
             Val a[3];
             int r = 0;
@@ -573,6 +578,21 @@ struct Evaluator {
                 vals[i] = load(t, off);
             } else if (n.op == XG_OP_CONST) {
                 vals[i] = consts[n.index];
+            } else if (n.op == XG_OP_GATHER) {
+                // index_view: the leaf's element at offset v from its own offset
+                // (v = the strided offset of a multi-index); outside the leaf's
+                // extent the element is 0, where xtensor leaves it undefined
+                const xg_tensor& t = e->leaves[n.index];
+                const int64_t v = cast_to<int64_t>(vals[n.args[0]]);
+                int64_t lo = 0, hi = 0;
+                bool empty = false;
+                for (int k = 0; k < t.ndim; ++k) {
+                    if (t.shape[k] == 0) empty = true;
+                    const int64_t ext = (std::max<int64_t>(t.shape[k], 1) - 1) * t.strides[k];
+                    (ext < 0 ? lo : hi) += ext;
+                }
+                if (!empty && v >= lo && v <= hi) vals[i] = load(t, t.offset + v);
+                else vals[i] = convert(make<int32_t>(0), value_dt_of_storage(t.dtype));
             } else {
                 Val a[3];
                 for (int k = 0; k < n.nargs; ++k) a[k] = vals[n.args[k]];

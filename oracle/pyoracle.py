@@ -100,7 +100,16 @@ class Oracle:
             arr, code = a
             self._keep.append(arr)
             return host_tensor(arr, code)
-        arr = self.leaves[id(a)]
+        arr = self.leaves.get(id(a))
+        if arr is None and id(getattr(a, "parent", None)) in self.leaves:
+            # a view the front end derived from a registered dense array: the
+            # same element offsets over that array's host copy
+            base = np.ascontiguousarray(self.leaves[id(a.parent)]).reshape(-1)
+            start = a.offset - a.parent.offset
+            arr = np.lib.stride_tricks.as_strided(base[start:], shape=a.shape,
+                                                  strides=[s * base.itemsize for s in a.strides])
+        if arr is None:
+            raise KeyError(f"no host data registered for {a!r}")
         self._keep.append(arr)
         return host_tensor(arr, a.dtype_code)
 

@@ -31,6 +31,7 @@ FILES = [
     "test_strided_assign.cpp",
     "test_extended_xmath_reducers.cpp",
     "test_xnpy.cpp",
+    "test_xindex_view.cpp",
 ]
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kat_arrays.json")
 

@@ -428,6 +428,32 @@ def test_mean_with_accumulator_type():
     assert r.dtype == np.float32 and r[()] == np.float32(7.0) / np.float32(3.0)
 
 
+def test_gather_index_view_kats():
+    """test_xindex_view.cpp:24-38, 167-181: index_view(e, {{1,1},{1,2},{2,2}})
+    reads e(1,1), e(1,2), e(2,2); filter(a, cond) in row- and column-major
+    order; positions outside the source read as 0 (the oracle's defined value
+    where xtensor's is undefined)."""
+    h = np.random.default_rng(0).random((3, 3))
+    e = host(h)
+    ix = host(np.array([[1, 1], [1, 2], [2, 2], [3, 0]], np.int64))
+    v = xt.index_view(e.dev, ix.dev)
+    got = oracle_for(e, ix).assign(v)
+    assert np.array_equal(got, [h[1, 1], h[1, 2], h[2, 2], 0.0])
+    a_h = np.array([[[1, 3], [2, 4]], [[5, 7], [6, 8]]], np.int32)
+    cond = np.array([[[1, 1], [0, 0]], [[1, 1], [0, 0]]], bool)
+    a = host(a_h)
+    for order, exp in (("C", [1, 3, 5, 7]), ("F", [1, 5, 3, 7])):
+        idx = np.argwhere(cond) if order == "C" else np.argwhere(cond.T)[:, ::-1].copy()
+        ixd = host(idx.astype(np.int64))
+        assert np.array_equal(oracle_for(a, ixd).assign(xt.index_view(a.dev, ixd.dev)), exp)
+    # a gather through a transposed (strided) source and inside arithmetic
+    t = host(h.T.copy())
+    tv = Dual(xt.transpose(e.dev), h.T)
+    ixt = host(np.array([[0, 2], [2, 1]], np.int64))
+    r = oracle_for(tv, ixt).assign(xt.index_view(tv.dev, ixt.dev) * 2.0)
+    assert np.array_equal(r, [h.T[0, 2] * 2, h.T[2, 1] * 2])
+
+
 @pytest.mark.parametrize("case", range(len(RESULT_TYPES)))
 def test_result_types(case):
     dt, fn, expect = RESULT_TYPES[case]

@@ -41,7 +41,7 @@ OP = dict(
     WHERE=40, CLIP=41,
     ABS=50, FABS=51, EXP=52, EXP2=53, EXPM1=54, LOG=55, LOG2=56, LOG10=57, LOG1P=58, SQRT=59,
     CBRT=60, SIN=61, COS=62, TAN=63, TANH=64, SINH=65, COSH=66, ASIN=67, ACOS=68, ATAN=69,
-    ERF=70, FLOOR=71, CEIL=72, TRUNC=73, ROUND=74,
+    ERF=70, FLOOR=71, CEIL=72, TRUNC=73, ROUND=74, GATHER=80,
 )
 
 # reductions / scans
@@ -150,6 +150,7 @@ EXPORTS = {
     "xg_reduce": (C.c_int, [_P(Expr), _P(ReduceDesc), _P(Tensor), C.c_void_p]),
     "xg_scan_info": (C.c_int, [_P(Expr), _P(ScanDesc), _P(C.c_int32), _P(C.c_int32), _P(C.c_int64)]),
     "xg_scan": (C.c_int, [_P(Expr), _P(ScanDesc), _P(Tensor), C.c_void_p]),
+    "xg_assign_indexed": (C.c_int, [_P(Expr), _P(Expr), _P(Tensor), C.c_void_p]),
     "xg_npy_info": (C.c_int, [C.c_char_p, _P(C.c_int32), _P(C.c_int32), _P(C.c_int64), _P(C.c_int32)]),
     "xg_npy_load": (C.c_int, [C.c_char_p, _P(Tensor), C.c_void_p]),
     "xg_npy_dump": (C.c_int, [C.c_char_p, _P(Tensor), C.c_void_p]),

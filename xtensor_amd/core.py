@@ -445,8 +445,12 @@ def transpose(a: DeviceArray, perm: Sequence[int] | None = None) -> DeviceArray:
     return DeviceArray(a.storage, a.dtype_code, [a.shape[p] for p in perm], [a.strides[p] for p in perm], a.offset)
 
 
-def broadcast(a: DeviceArray, shape: Sequence[int]) -> DeviceArray:
-    """xt::broadcast of a container to `shape`: stride-0 view."""
+def broadcast(a, shape: Sequence[int]):
+    """xt::broadcast of a container to `shape`: stride-0 view.  A scalar or an
+    expression is returned as it is: assignment broadcasts it to the target
+    (views/xbroadcast.hpp:157-480 makes the same values available lazily)."""
+    if not isinstance(a, DeviceArray):
+        return _wrap(a)
     shape = tuple(int(s) for s in shape)
     if len(shape) < a.ndim:
         raise BroadcastError("broadcast: target rank smaller than array rank")
@@ -625,13 +629,16 @@ def serialize(root: Expr, leaf_fn: Callable[[DeviceArray], A.Tensor] | None = No
             raise NotImplementedError("expression has too many nodes")
         return len(nodes) - 1
 
-    def leaf(arr):
+    def leaf_slot(arr):
         k = id(arr)
         if k not in leaf_of:
             leaf_of[k] = len(leaves)
             leaves.append(leaf_fn(arr))
             keep.append(arr)
-        return add(A.OP["LEAF"], (), leaf_of[k])
+        return leaf_of[k]
+
+    def leaf(arr):
+        return add(A.OP["LEAF"], (), leaf_slot(arr))
 
     def visit(e):
         k = id(e)
@@ -639,6 +646,9 @@ def serialize(root: Expr, leaf_fn: Callable[[DeviceArray], A.Tensor] | None = No
             return memo[k]
         if isinstance(e, DeviceArray):
             r = leaf(e)
+        elif isinstance(e, IndexView):  # a gather of the source at the offsets' values
+            off = visit(e.offsets)
+            r = add(A.OP["GATHER"], (off,), leaf_slot(e.source))
         elif isinstance(e, Reducer):
             if reducer_fn is None:
                 raise RuntimeError("reducer inside an expression needs an evaluation context")
@@ -723,6 +733,9 @@ def _leaves(e) -> list:
                 walk(y)
         elif isinstance(x, Reducer):
             walk(x.expr)
+        elif isinstance(x, IndexView):
+            out.append(x.source)
+            walk(x.offsets)
     walk(e)
     return out
 
@@ -742,9 +755,11 @@ def _reduce_into(r: Reducer, out: DeviceArray) -> None:
     _check(A.lib().xg_reduce(s.ptr(), C.byref(d), C.byref(t), _stream()))
 
 
-def assign(out: DeviceArray, e) -> DeviceArray:
+def assign(out, e):
     """xt::noalias(out) = e (or assignment into a view).  Returns out."""
     e = _wrap(e)
+    if isinstance(out, IndexView):
+        return out.assign(e)
     if isinstance(e, Reducer):
         _reduce_into(e, out)
         return out
@@ -803,6 +818,128 @@ def eval(e) -> DeviceArray:  # noqa: A001
     dt, shp = expr_info(e)
     out = empty(shp, dt)
     return assign(out, e)
+
+
+# ----------------------------------------------------------------------------
+# index views, filters, filtrations, masked views (views/xindex_view.hpp,
+# views/xmasked_view.hpp)
+class IndexView(Expr):
+    """xt::index_view(e, indices) (views/xindex_view.hpp:786-797): the 1-d
+    sequence of e's elements at `indices` — multi-indices of shape
+    [n, e.ndim] (or positions [n] for a 1-d e), on the host or the device.
+    Read as an operand it is a gather (one fused launch with the rest of the
+    expression); assigned into, it scatters.  Indices outside e read as 0 and
+    are skipped on assignment (xtensor leaves them undefined)."""
+
+    def __init__(self, source, indices):
+        if not isinstance(source, DeviceArray):  # a function: index its evaluated values (read-only)
+            source = eval(_wrap(source))
+        self.source = source
+        nd = len(source.shape)
+        if isinstance(indices, DeviceArray):
+            idx = indices
+            shape = tuple(idx.shape)
+        else:
+            h = np.asarray(indices, dtype=np.int64)
+            if h.ndim == 1 and nd != 1:
+                h = h.reshape(-1, nd) if h.size % max(nd, 1) == 0 and h.size else h.reshape(0, nd)
+            idx = from_numpy(h)
+            shape = h.shape
+        self.index = idx  # the index array as given (or uploaded)
+        if len(shape) == 1:
+            if nd != 1:
+                raise ValueError("positions need a 1-d source; give [n, ndim] multi-indices")
+            shape = (shape[0], 1)
+            idx = DeviceArray(idx.storage, idx.dtype, shape, (idx.strides[0], 0), idx.offset)
+        if shape[1] != nd:
+            raise ValueError(f"indices have {shape[1]} components for a {nd}-d source")
+        self.n = int(shape[0])
+        # offset of element i: sum over k of indices[i, k] * stride_k
+        off = None
+        for k in range(nd):
+            col = DeviceArray(idx.storage, idx.dtype, (self.n,), (idx.strides[0],), idx.offset + k * idx.strides[1])
+            col.parent = self.index  # a view of the index array (lets a checker map it to host data)
+            term = col * scalar(int(source.strides[k]), "int64")
+            off = term if off is None else off + term
+        self.offsets = off if off is not None else scalar(0, "int64")
+        if nd == 0:
+            raise ValueError("index_view of a 0-d array")
+
+    def fill(self, value):
+        """xindex_view::fill: every indexed element = value."""
+        return self.assign(value)
+
+    def assign(self, e):
+        """index_view(e, indices) = values: a scatter of the evaluated values."""
+        e = _wrap(e)
+        _require_device(_leaves(e) + _leaves(self.offsets) + [self.source])
+        sv = serialize(e, reducer_fn=_eval_reducer)
+        so = serialize(self.offsets)
+        t = self.source.to_abi()
+        _check(A.lib().xg_assign_indexed(sv.ptr(), so.ptr(), C.byref(t), _stream()))
+        return self
+
+
+def index_view(e, indices) -> IndexView:
+    return IndexView(e, indices)
+
+
+def argwhere(cond, layout="row_major") -> np.ndarray:
+    """xt::argwhere<L>: multi-indices of the true elements in row-major (or
+    column-major) traversal order (the condition is evaluated on the device,
+    the positions listed on the host)."""
+    c = eval(cast(_wrap(cond), "bool")).to_numpy()
+    if layout == "column_major":
+        return np.argwhere(c.T)[:, ::-1].copy()
+    return np.argwhere(c)
+
+
+def filter(e, cond, layout="row_major") -> IndexView:  # noqa: A001
+    """xt::filter<L>(e, cond) = index_view(e, argwhere<L>(cond)) (views/xindex_view.hpp:820-826)."""
+    return IndexView(e, argwhere(cond, layout))
+
+
+class Filtration:
+    """xt::filtration(e, cond) (views/xindex_view.hpp:845-850): scalar (and
+    computed) assignment to the elements of e where cond holds — on the
+    device e = where(cond, value, e), one fused launch, e read and written
+    element by element."""
+
+    def __init__(self, e, cond):
+        if not isinstance(e, DeviceArray):
+            raise TypeError("filtration needs a device array or a strided view of one")
+        self.e = e
+        self.cond = _wrap(cond)
+
+    def assign(self, v):
+        return assign(self.e, where(self.cond, _wrap(v), self.e))
+
+    def __iadd__(self, v):
+        assign(self.e, where(self.cond, self.e + _wrap(v), self.e))
+        return self
+
+    def __isub__(self, v):
+        assign(self.e, where(self.cond, self.e - _wrap(v), self.e))
+        return self
+
+    def __imul__(self, v):
+        assign(self.e, where(self.cond, self.e * _wrap(v), self.e))
+        return self
+
+    def __itruediv__(self, v):
+        assign(self.e, where(self.cond, self.e / _wrap(v), self.e))
+        return self
+
+
+def filtration(e, cond) -> Filtration:
+    return Filtration(e, cond)
+
+
+def masked_view(e, mask) -> Filtration:
+    """xt::masked_view(e, mask) (views/xmasked_view.hpp) for assignment:
+    `masked_view(a, m).assign(v)` sets a's elements where the (broadcast)
+    mask is true; reading masked values is not offered on the device."""
+    return Filtration(e, mask)
 
 
 # ----------------------------------------------------------------------------

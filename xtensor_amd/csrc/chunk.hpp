@@ -64,6 +64,9 @@ __device__ __forceinline__ void run_static(const LeafView* leaves, const uint64_
         if constexpr (m.op == MO_LOAD) {
             load_leaf<V, I64>(leaves[m.aux], S::leaf_mode[m.aux] | (leaves[m.aux].mode & LM_NT), S::leaf_dtype[m.aux], idx, rank, j0, nvalid,
                               R[m.dst]);
+        } else if constexpr (m.op == MO_GATHER) {
+            const LeafView& lv = leaves[m.aux];
+            gather_vals<V>(lv.ptr, S::leaf_dtype[m.aux], lv.stride[0], lv.stride[1], nvalid, R[m.a], R[m.dst]);
         } else {
             Reg tmp[V];
 #pragma unroll
@@ -97,6 +100,9 @@ __device__ __forceinline__ void eval_chunk(const LeafView* leaves, const Program
             if (m.op == MO_LOAD) {
                 const LeafView& lv = leaves[m.aux];
                 load_leaf<V, I64>(lv, lv.mode, lv.dtype, idx, rank, j0, nvalid, tmp);
+            } else if (m.op == MO_GATHER) {
+                const LeafView& lv = leaves[m.aux];
+                gather_vals<V>(lv.ptr, lv.dtype, lv.stride[0], lv.stride[1], nvalid, R[m.a], tmp);
             } else {
 #pragma unroll
                 for (int v = 0; v < V; ++v) tmp[v] = exec_scalar_dyn(m, R[m.a][v], R[m.b][v], R[m.c][v], prog.consts);

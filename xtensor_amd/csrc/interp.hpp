@@ -313,6 +313,23 @@ __device__ __forceinline__ void load_vec(const char* p, int esz, int dt, Reg (&o
     }
 }
 
+// MO_GATHER: element offsets `off` (int64) into a tensor at `base`; offsets
+// outside [lo, hi] (and lanes past nvalid) read as 0, so a bad index never
+// touches memory outside the tensor
+template <int V>
+__device__ __forceinline__ void gather_vals(const char* base, int dt, int64_t lo, int64_t hi, int nvalid,
+                                            const Reg (&off)[V], Reg (&out)[V]) {
+    const int esz = dtype_size(dt);
+    Reg tmp[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int64_t o = int64_t(off[v]);
+        tmp[v] = (v < nvalid && o >= lo && o <= hi) ? from_storage(load_raw(base + o * esz, esz), dt) : Reg(0);
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) out[v] = tmp[v];
+}
+
 // ---------------------------------------------------------------- stores
 __device__ __forceinline__ uint64_t to_storage(Reg r, int dt) {
     if (dt == XG_BFLOAT16) {  // float -> bf16, round to nearest even, NaN kept quiet

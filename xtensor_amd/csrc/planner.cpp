@@ -235,6 +235,12 @@ ExprInfo analyse(const xg_expr* e) {
             if (n.nargs != 0 || n.index < 0 || n.index >= e->n_leaves) raise(XG_INVALID_ARGUMENT, "bad leaf node");
             info.node_dt[size_t(i)] = value_dtype(e->leaves[n.index].dtype);
             rank[size_t(i)] = e->leaves[n.index].ndim;
+        } else if (n.op == XG_OP_GATHER) {
+            if (n.nargs != 1 || n.index < 0 || n.index >= e->n_leaves) raise(XG_INVALID_ARGUMENT, "bad gather node");
+            const int odt = info.node_dt[size_t(n.args[0])];
+            if (!is_int_dt(odt) || odt == XG_BOOL) raise(XG_INVALID_ARGUMENT, "gather offsets must be integers");
+            info.node_dt[size_t(i)] = value_dtype(e->leaves[n.index].dtype);
+            rank[size_t(i)] = rank[size_t(n.args[0])];
         } else if (n.op == XG_OP_CONST) {
             if (n.nargs != 0 || n.index < 0 || n.index >= e->n_consts) raise(XG_INVALID_ARGUMENT, "bad constant node");
             info.node_dt[size_t(i)] = e->consts[n.index].dtype;
@@ -291,11 +297,14 @@ struct Lowerer {
     std::vector<int> leaf_ssa;
     std::vector<int> const_ssa;
     std::vector<int> leaf_order;   // expression leaf number of program leaf k
+    std::vector<uint8_t> leaf_is_gather;
+    std::vector<int> gather_slot;  // program leaf of an expression leaf read by MO_GATHER
     std::vector<int> const_order;
 
     Lowerer(const xg_expr* ex, const ExprInfo& in) : e(ex), info(in) {
         node_ssa.assign(size_t(e->n_nodes), -1);
         leaf_ssa.assign(size_t(e->n_leaves), -1);
+        gather_slot.assign(size_t(e->n_leaves), -1);
         const_ssa.assign(size_t(e->n_consts), -1);
     }
 
@@ -328,10 +337,23 @@ struct Lowerer {
         if (n.op == XG_OP_LEAF) {
             int& ls = leaf_ssa[size_t(n.index)];
             if (ls < 0) {
+                if (leaf_order.size() >= size_t(kMaxLeaves)) raise(XG_UNSUPPORTED, "too many leaves");
                 ls = emit(MO_LOAD, e->leaves[n.index].dtype, rdt, -1, -1, -1, int(leaf_order.size()));
                 leaf_order.push_back(n.index);
+                leaf_is_gather.push_back(0);
             }
             return s = ls;
+        }
+        if (n.op == XG_OP_GATHER) {
+            const int off = cvt(value(n.args[0]), XG_INT64);
+            int& slot = gather_slot[size_t(n.index)];
+            if (slot < 0) {
+                if (leaf_order.size() >= size_t(kMaxLeaves)) raise(XG_UNSUPPORTED, "too many leaves");
+                slot = int(leaf_order.size());
+                leaf_order.push_back(n.index);
+                leaf_is_gather.push_back(1);
+            }
+            return s = emit(MO_GATHER, e->leaves[n.index].dtype, rdt, off, -1, -1, slot);
         }
         if (n.op == XG_OP_CONST) {
             int& cs = const_ssa[size_t(n.index)];
@@ -494,7 +516,10 @@ Program lower(const xg_expr* e, const ExprInfo& info, int final_dt, int map) {
     p.nregs = std::max(nregs, 1);
     p.nleaves = int(L.leaf_order.size());
     p.nconsts = int(L.const_order.size());
-    for (int k = 0; k < p.nleaves; ++k) p.leaf_src[k] = int8_t(L.leaf_order[size_t(k)]);
+    for (int k = 0; k < p.nleaves; ++k) {
+        p.leaf_src[k] = int8_t(L.leaf_order[size_t(k)]);
+        p.leaf_gather[k] = L.leaf_is_gather[size_t(k)];
+    }
     for (int k = 0; k < p.nconsts; ++k) p.const_src[k] = int8_t(L.const_order[size_t(k)]);
     p.result_reg = reg[size_t(root)];
     p.result_dtype = rdt;
@@ -552,6 +577,13 @@ EwPlan plan_assign(const xg_expr* e, const xg_tensor* out, int V) {
     p.prog = lower(e, info, out->dtype);
     p.out_dtype = out->dtype;
     auto ls = leaf_strides(e, os);
+    {  // leaves read only through MO_GATHER are not iterated: no strides
+        std::vector<bool> iterated(size_t(e->n_leaves), false);
+        for (int pl = 0; pl < p.prog.nleaves; ++pl)
+            if (!p.prog.leaf_gather[pl]) iterated[size_t(p.prog.leaf_src[pl])] = true;
+        for (int l = 0; l < e->n_leaves; ++l)
+            if (!iterated[size_t(l)]) std::fill(ls[size_t(l)].begin(), ls[size_t(l)].end(), 0);
+    }
     // collect dims, dropping size-1 ones
     std::vector<Dim> dims;
     int64_t total = numel(os);
@@ -634,6 +666,10 @@ EwPlan plan_assign(const xg_expr* e, const xg_tensor* out, int V) {
         int l = p.prog.leaf_src[pl];
         LeafView& lv = p.leaves[pl];
         const xg_tensor& t = e->leaves[l];
+        if (p.prog.leaf_gather[pl]) {  // read at computed offsets: the tensor's whole extent
+            set_gather_leaf(t, &lv);
+            continue;
+        }
         int esz = dtype_size(t.dtype);
         lv.ptr = static_cast<const char*>(t.data) + leaf_off[size_t(l)] * esz;
         lv.dtype = t.dtype;
@@ -653,7 +689,8 @@ EwPlan plan_assign(const xg_expr* e, const xg_tensor* out, int V) {
         for (int k = 0; k < rank; ++k) {
             p.shape32[k] = uint32_t(p.shape[k]);
             p.cst[kMaxLeaves][k] = int32_t(p.out_stride[k]);
-            for (int pl = 0; pl < p.prog.nleaves; ++pl) p.cst[pl][k] = int32_t(p.leaves[pl].stride[k]);
+            for (int pl = 0; pl < p.prog.nleaves; ++pl)
+                p.cst[pl][k] = p.prog.leaf_gather[pl] ? 0 : int32_t(p.leaves[pl].stride[k]);
         }
     }
     return p;
@@ -662,6 +699,22 @@ EwPlan plan_assign(const xg_expr* e, const xg_tensor* out, int V) {
 void set_nontemporal(int loads, int stores) {
     g_nt_loads = loads;
     g_nt_stores = stores;
+}
+
+void set_gather_leaf(const xg_tensor& t, LeafView* lv) {
+    int64_t lo = 0, hi = 0;
+    bool empty = false;
+    for (int k = 0; k < t.ndim; ++k) {
+        if (t.shape[k] == 0) empty = true;
+        const int64_t ext = (std::max<int64_t>(t.shape[k], 1) - 1) * t.strides[k];
+        (ext < 0 ? lo : hi) += ext;
+    }
+    for (int k = 0; k < kMaxRank; ++k) lv->stride[k] = 0;
+    lv->ptr = static_cast<const char*>(t.data) + t.offset * dtype_size(t.dtype);
+    lv->dtype = t.dtype;
+    lv->stride[0] = empty ? 1 : lo;  // an empty tensor has no valid offset
+    lv->stride[1] = empty ? 0 : hi;
+    lv->mode = LM_GATHER;
 }
 
 bool streams(const int64_t* stride, const int64_t* shape, int rank, int esz) {

@@ -72,6 +72,9 @@ SpecKey spec_key(const EwPlan& p);
 // this shape reads every element once (no stride-0 dim of extent > 1) and
 // spans at least kNtMinBytes: its loads then stream non-temporally.
 bool streams(const int64_t* stride, const int64_t* shape, int rank, int esz);
+// LeafView of a leaf read by MO_GATHER: base pointer (offset applied), mode
+// LM_GATHER, stride[0] / stride[1] = the lowest / highest valid element offset.
+void set_gather_leaf(const xg_tensor& t, LeafView* lv);
 // Switch non-temporal leaf loads / output stores on or off (measurement aid).
 void set_nontemporal(int loads, int stores);
 bool spec_equal(const SpecKey& a, const SpecKey& b);

@@ -375,7 +375,7 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
         d.red = out_dim_of[size_t(k)] < 0;
         int od = r->keep_dims ? k : out_dim_of[size_t(k)];
         d.out = d.red ? 0 : (out->shape[od] == 1 ? 0 : out->strides[od]);
-        for (int pl = 0; pl < nl; ++pl) d.leaf[pl] = ls[size_t(p.prog.leaf_src[pl])][size_t(k)];
+        for (int pl = 0; pl < nl; ++pl) d.leaf[pl] = p.prog.leaf_gather[pl] ? 0 : ls[size_t(p.prog.leaf_src[pl])][size_t(k)];
         dims.push_back(d);
     }
     if (empty_red) {  // keep the kept dims for output addressing, reduce nothing
@@ -484,6 +484,10 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
     for (int pl = 0; pl < nl; ++pl) {
         LeafView& lv = p.leaves[pl];
         const xg_tensor& tl = e->leaves[p.prog.leaf_src[pl]];
+        if (p.prog.leaf_gather[pl]) {
+            set_gather_leaf(tl, &lv);
+            continue;
+        }
         int esz = dtype_size(tl.dtype);
         lv.ptr = static_cast<const char*>(tl.data) + leaf_off[size_t(pl)] * esz;
         lv.dtype = tl.dtype;
@@ -543,7 +547,7 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
     if (!p.index64 && (rows_ok || cols_ok) && !empty_red) {
         p.compact = 1;
         for (int pl = 0; pl < nl; ++pl)
-            for (int k = 0; k < p.rank; ++k) p.cst[pl][k] = int32_t(p.leaves[pl].stride[k]);
+            for (int k = 0; k < p.rank; ++k) p.cst[pl][k] = p.prog.leaf_gather[pl] ? 0 : int32_t(p.leaves[pl].stride[k]);
         const uint64_t target_waves = uint64_t(cus) * 16;  // enough loads in flight to cover HBM latency
         constexpr uint64_t kU = 4;                          // chunks (rows) in flight per lane
         if (p.layout == RL_ROWS) {

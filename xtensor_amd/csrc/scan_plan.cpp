@@ -101,7 +101,7 @@ ScanPlan plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out
         SDim d{};
         d.n = info.shape[size_t(k)];
         d.out = out->shape[k] == 1 ? 0 : out->strides[k];
-        for (int pl = 0; pl < nl; ++pl) d.leaf[pl] = ls[size_t(p.prog.leaf_src[pl])][size_t(k)];
+        for (int pl = 0; pl < nl; ++pl) d.leaf[pl] = p.prog.leaf_gather[pl] ? 0 : ls[size_t(p.prog.leaf_src[pl])][size_t(k)];
         if (k == axis) ad = d;
         else if (d.n != 1) (k < axis ? before : after).push_back(d);
     }
@@ -160,6 +160,10 @@ ScanPlan plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out
     for (int pl = 0; pl < nl; ++pl) {
         LeafView& lv = p.leaves[pl];
         const xg_tensor& t = e->leaves[p.prog.leaf_src[pl]];
+        if (p.prog.leaf_gather[pl]) {
+            set_gather_leaf(t, &lv);
+            continue;
+        }
         int esz = dtype_size(t.dtype);
         lv.ptr = static_cast<const char*>(t.data) + t.offset * esz;
         lv.dtype = t.dtype;

@@ -69,6 +69,9 @@ __device__ __forceinline__ void run_spec(const P& plan, const uint32_t (&idx)[4]
                 load_at<V, true>(p, mode, dt, plan.cst[m.aux][RANK - 1], nvalid, R[m.dst]);
             else
                 load_at<V>(p, mode, dt, plan.cst[m.aux][RANK - 1], nvalid, R[m.dst]);
+        } else if constexpr (m.op == MO_GATHER) {
+            gather_vals<V>(leaf_ptr(plan, m.aux), S::leaf_dtype[m.aux], plan.leaves[m.aux].stride[0],
+                           plan.leaves[m.aux].stride[1], nvalid, R[m.a], R[m.dst]);
         } else {
             Reg tmp[V];
 #pragma unroll

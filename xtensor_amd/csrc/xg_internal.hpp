@@ -56,6 +56,7 @@ enum MicroOpCode : uint8_t {
     MO_EXP, MO_EXP2, MO_EXPM1, MO_LOG, MO_LOG2, MO_LOG10, MO_LOG1P, MO_SQRT, MO_CBRT,
     MO_SIN, MO_COS, MO_TAN, MO_TANH, MO_SINH, MO_COSH, MO_ASIN, MO_ACOS, MO_ATAN, MO_ERF,
     MO_FLOOR, MO_CEIL, MO_TRUNC, MO_ROUND,
+    MO_GATHER,      // dst <- leaf[aux] at element offset a (int64), storage type t; 0 outside its extent
     MO_NUM_CODES
 };
 
@@ -80,6 +81,7 @@ enum LeafMode : uint8_t {
     LM_CONTIG = 0,  // inner stride 1 and vector-aligned: wide loads
     LM_BCAST = 1,   // inner stride 0: one load per chunk
     LM_STRIDED = 2, // anything else: one load per element
+    LM_GATHER = 3,  // read at computed offsets (MO_GATHER); stride[0..1] = valid offset range
     // flag on CONTIG / STRIDED: a large leaf whose every element is read by
     // exactly one work item (no broadcast reuse) is loaded non-temporally,
     // so the stream does not evict cached lines (dirty ones included) that
@@ -137,6 +139,7 @@ struct Program {
     // the program numbers leaves and constants in order of first use; these
     // map that numbering back to the expression's leaf / constant numbers
     int8_t leaf_src[kMaxLeaves];
+    uint8_t leaf_gather[kMaxLeaves];  // 1: the leaf is read by MO_GATHER only
     int8_t const_src[kMaxConsts];
     MicroOp ops[kMaxOps];
     uint64_t consts[kMaxConsts];
