@@ -20,6 +20,7 @@ from .core import (  # noqa: F401
     tanh, transpose, trunc, variance, view, where, zeros,
 )
 
+from . import blockwise  # noqa: F401,E402
 from .batch import Batch  # noqa: F401,E402
 from .core import execute_many  # noqa: F401,E402
 from .serve import Server  # noqa: F401,E402
