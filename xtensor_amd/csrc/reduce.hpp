@@ -105,6 +105,10 @@ hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** use
 
 size_t reduce_workspace_bytes(const RedPlan& p);
 
+// Waves per CU the compiled reduction kernels' work split aims at (default
+// 16; measurement aid), for plans made after the call.
+void set_reduce_waves_per_cu(int w);
+
 struct RSpecEntry;
 // compiled specialisation matching the plan, or nullptr
 const RSpecEntry* find_rspec(const RedPlan& p, int V);

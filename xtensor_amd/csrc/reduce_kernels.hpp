@@ -395,6 +395,9 @@ __global__ __launch_bounds__(64 * kColsWaves) void reduce_cols_fast_kernel(const
     __shared__ Store lds[W][64][V];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    // split fastest: the splits of one column strip run side by side (measured
+    // faster than tile-fastest, which reads whole rows per group of blocks:
+    // 3.6 vs 3.1 TB/s on a [8192, 4096] float32 column sum)
     const uint32_t tile = fastdiv(blockIdx.x, p.div_splits);
     const uint32_t sp = blockIdx.x - tile * uint32_t(p.splits);
     const uint32_t cc = tile * 64 + uint32_t(lane);

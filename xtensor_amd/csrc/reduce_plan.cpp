@@ -16,6 +16,9 @@
 
 namespace xg {
 
+int g_reduce_waves_per_cu = 16;
+void set_reduce_waves_per_cu(int w) { g_reduce_waves_per_cu = w > 0 ? w : 16; }
+
 namespace {
 
 int big_promote(int dt) {  // restatement of xtl::big_promote_type_t for arithmetic types
@@ -548,7 +551,8 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
         p.compact = 1;
         for (int pl = 0; pl < nl; ++pl)
             for (int k = 0; k < p.rank; ++k) p.cst[pl][k] = p.prog.leaf_gather[pl] ? 0 : int32_t(p.leaves[pl].stride[k]);
-        const uint64_t target_waves = uint64_t(cus) * 16;  // enough loads in flight to cover HBM latency
+        // enough loads in flight to cover HBM latency
+        const uint64_t target_waves = uint64_t(cus) * uint64_t(g_reduce_waves_per_cu);
         constexpr uint64_t kU = 4;                          // chunks (rows) in flight per lane
         if (p.layout == RL_ROWS) {
             const uint64_t chunks = p.nchunks_f;
