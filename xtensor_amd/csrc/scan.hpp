@@ -23,7 +23,7 @@ struct ScanPlan {
     int32_t rdt;        // result type R
     int32_t out_dtype;
     int32_t index64;
-    int32_t pad_;
+    int32_t out_mode;   // LM_CONTIG or LM_STRIDED, | LM_NT for large outputs
     int64_t shape[kMaxRank];
     FastDiv div[kMaxRank];
     uint32_t rows;      // SL_ROWS: rows; SL_COLS: outer * ceil(inner / V)

@@ -28,7 +28,7 @@ __device__ __forceinline__ void scan_store(const ScanPlan& p, const uint32_t* id
     Reg o[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) o[v] = odt == XG_BFLOAT16 ? cvt(val[v], p.rdt, XG_FLOAT32) : cvt(val[v], p.rdt, odt);
-    if (p.pad_ == LM_CONTIG && nvalid == V) {
+    if (lm_base(p.out_mode) == LM_CONTIG && nvalid == V) {
         store_vec<V>(ptr, esz, odt, o);
     } else {
         const int64_t os = p.out_stride[p.rank - 1];
@@ -174,25 +174,27 @@ __global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p)
     using T = typename ScanIn<AK>::T;
     constexpr int V = sizeof(T) == 4 ? 4 : 2;
     constexpr int U = 4;
+    constexpr uint32_t BLK = 64u * V * U;  // elements of one block of U segments
     const int lane = threadIdx.x & 63;
     const uint32_t row = blockIdx.x * 4u + uint32_t(threadIdx.x >> 6);
     if (row >= p.rows) return;  // whole waves leave: row is uniform in a wave
     const T* in = reinterpret_cast<const T*>(p.leaves[0].ptr) + int64_t(row) * p.leaves[0].stride[0];
+    const bool nt_in = lm_nt(p.leaves[0].mode), nt_out = lm_nt(p.out_mode);
     const int odt = p.out_dtype;
     const int oesz = dtype_size(odt);
     char* out = p.out_ptr + int64_t(row) * p.out_stride[0] * oesz;
     A carry;
     carry.init(p.identity);
-    for (uint32_t base = 0; base < p.len; base += 64u * V * U) {
-        T x[U][V];
-        int nv[U];
+    // load block `base` (U segments of 64 * V elements) into x / nv
+    auto load = [&](uint32_t base, T (&x)[U][V], int (&nv)[U]) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t j = base + uint32_t(u * 64 * V) + uint32_t(lane * V);
             nv[u] = j < p.len ? min(V, int(p.len - j)) : 0;
             if (nv[u] == V) {
                 Reg r[V];
-                load_vec<V>(reinterpret_cast<const char*>(in + j), int(sizeof(T)), ScanInDtype<T>::value, r);
+                if (nt_in) load_vec<V, true>(reinterpret_cast<const char*>(in + j), int(sizeof(T)), ScanInDtype<T>::value, r);
+                else load_vec<V>(reinterpret_cast<const char*>(in + j), int(sizeof(T)), ScanInDtype<T>::value, r);
 #pragma unroll
                 for (int v = 0; v < V; ++v) x[u][v] = reg_get<T>(r[v]);
             } else {
@@ -200,6 +202,9 @@ __global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p)
                 for (int v = 0; v < V; ++v) x[u][v] = v < nv[u] ? in[j + uint32_t(v)] : T(0);
             }
         }
+    };
+    // scan block `base` from x / nv, store it, carry on
+    auto process = [&](uint32_t base, const T (&x)[U][V], const int (&nv)[U]) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (base + uint32_t(u * 64 * V) >= p.len) break;  // uniform
@@ -234,7 +239,8 @@ __global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p)
             const uint32_t j = base + uint32_t(u * 64 * V) + uint32_t(lane * V);
             char* ptr = out + int64_t(j) * oesz;
             if (nv[u] == V) {
-                store_vec<V>(ptr, oesz, odt, o);
+                if (nt_out) store_vec<V, true>(ptr, oesz, odt, o);
+                else store_vec<V>(ptr, oesz, odt, o);
             } else {
 #pragma unroll
                 for (int v = 0; v < V; ++v)
@@ -242,6 +248,14 @@ __global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p)
             }
             carry.merge(incl.shfl_idx(63));
         }
+    };
+    // one register set (a second one, to overlap the next block's loads with
+    // this block's scan, measured slower: 5 instead of 8 waves per SIMD)
+    for (uint32_t base = 0; base < p.len; base += BLK) {
+        T x[U][V];
+        int nv[U];
+        load(base, x, nv);
+        process(base, x, nv);
     }
 }
 

@@ -156,7 +156,12 @@ ScanPlan plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out
     bool oal = p.out_stride[p.rank - 1] == 1 && reinterpret_cast<uintptr_t>(p.out_ptr) % uintptr_t(osz * V) == 0;
     for (int k = 0; k + 1 < p.rank; ++k)
         if (p.out_stride[k] % V != 0) oal = false;
-    p.pad_ = oal ? LM_CONTIG : LM_STRIDED;
+    p.out_mode = oal ? LM_CONTIG : LM_STRIDED;
+    {
+        int64_t total = 1;
+        for (int k = 0; k < p.rank; ++k) total *= std::max<int64_t>(p.shape[k], 0);
+        if (total * osz >= kNtMinBytes) p.out_mode |= LM_NT;
+    }
     for (int pl = 0; pl < nl; ++pl) {
         LeafView& lv = p.leaves[pl];
         const xg_tensor& t = e->leaves[p.prog.leaf_src[pl]];
@@ -177,6 +182,7 @@ ScanPlan plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out
         max_off = std::max(max_off, lmax);
         int64_t is = lv.stride[p.rank - 1];
         lv.mode = is == 0 ? LM_BCAST : (is == 1 && al) ? LM_CONTIG : LM_STRIDED;
+        if (lv.mode != LM_BCAST && streams(lv.stride, p.shape, p.rank, esz)) lv.mode |= LM_NT;
     }
     p.index64 = max_off >= (int64_t(1) << 31) ? 1 : 0;
     return p;
