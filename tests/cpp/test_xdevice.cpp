@@ -209,6 +209,22 @@ static void gpu_tests() {
     EXPECT(mm.size() == 2 && mm[0] == *std::min_element(hi.begin(), hi.end()) &&
            mm[1] == *std::max_element(hi.begin(), hi.end()));
 
+    // index views, filter, filtration (test_xindex_view.cpp:24-57, 140-156)
+    std::vector<double> he = {0.5, 1.5, 2.5, 3.5, 4.5, 5.5, 6.5, 7.5, 8.5};
+    auto e = xt::xdevice_array<double>::from_host(he, {3, 3});
+    auto iv = xt::index_view(e, {{1, 1}, {1, 2}, {2, 2}});
+    auto ivh = xt::eval<double>(iv).to_host();
+    EXPECT(ivh == std::vector<double>({4.5, 5.5, 8.5}));
+    iv += 3;
+    auto eh = e.to_host();
+    EXPECT(eh[4] == 7.5 && eh[5] == 8.5 && eh[8] == 11.5 && eh[0] == 0.5);
+    auto a5 = xt::xdevice_array<double>::from_host({1, 5, 3, 4, 5, 6}, {2, 3});
+    xt::filter(a5, a5 >= 5) = 100;
+    EXPECT(a5.to_host() == std::vector<double>({1, 100, 3, 4, 100, 100}));
+    auto b5 = xt::xdevice_array<double>::from_host({1, 5, 3, 4, 5, 6}, {2, 3});
+    xt::filtration(b5, b5 >= 5) += 2;
+    EXPECT(b5.to_host() == std::vector<double>({1, 7, 3, 4, 7, 8}));
+
     // cumsum along an axis (reducers/xaccumulator.hpp:282-294): exact on small integers
     std::vector<double> hsmall(6 * 5);
     for (size_t k = 0; k < hsmall.size(); ++k) hsmall[k] = double(int(k % 7) - 3);

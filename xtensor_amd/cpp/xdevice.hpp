@@ -89,8 +89,12 @@ using shape_type = std::vector<int64_t>;
 class descriptor {
   public:
     int leaf(const xg_tensor& t) {
+        return node(XG_OP_LEAF, {}, leaf_slot(t));
+    }
+    // a leaf that no node reads directly (the source of a gather)
+    int leaf_slot(const xg_tensor& t) {
         leaves_.push_back(t);
-        return node(XG_OP_LEAF, {}, int(leaves_.size()) - 1);
+        return int(leaves_.size()) - 1;
     }
     template <class T> int constant(T v) {
         xg_scalar s{};
@@ -608,6 +612,111 @@ template <class T, class E> xdevice_array<T> scan(int op, const E& e, int axis) 
 }
 template <class T, class E> xdevice_array<T> cumsum(const E& e, int axis) { return scan<T>(XG_SCAN_CUMSUM, e, axis); }
 template <class T, class E> xdevice_array<T> cumprod(const E& e, int axis) { return scan<T>(XG_SCAN_CUMPROD, e, axis); }
+
+// ---------------------------------------------------------------- index views
+// xt::index_view(e, indices) (views/xindex_view.hpp:786-797): the 1-d
+// sequence of e's elements at the multi-indices `indices` ([n, e.dimension()],
+// int64, on the device).  As an operand it is a gather fused into the
+// expression; assigning to it scatters (xg_assign_indexed).  Indices outside
+// e read as 0 and are skipped on assignment.
+template <class T> struct xdev_index_view : xdev_expression<xdev_index_view<T>> {
+    using value_type = T;
+    xdevice_array<T> source;
+    xdevice_array<int64_t> index;
+
+    xdev_index_view(xdevice_array<T> s, xdevice_array<int64_t> i) : source(std::move(s)), index(std::move(i)) {
+        if (index.dimension() != 2 || size_t(index.shape()[1]) != source.dimension() || source.dimension() == 0)
+            throw std::invalid_argument("index_view: indices must be [n, e.dimension()]");
+    }
+    int64_t size() const { return index.shape()[0]; }
+    // element offsets: sum over k of indices[:, k] * stride_k
+    int offsets(descriptor& d) const {
+        int off = -1;
+        for (size_t k = 0; k < source.dimension(); ++k) {
+            auto col = index.reshaped_view({index.shape()[0]}, {index.strides()[0]},
+                                           index.offset() + int64_t(k) * index.strides()[1]);
+            const int t = d.node(XG_OP_MULTIPLIES, {col.flatten(d), d.constant(int64_t(source.strides()[k]))});
+            off = off < 0 ? t : d.node(XG_OP_PLUS, {off, t});
+        }
+        return off;
+    }
+    int flatten(descriptor& d) const {
+        const int off = offsets(d);
+        return d.node(XG_OP_GATHER, {off}, d.leaf_slot(source.tensor()));
+    }
+    template <class E> xdev_index_view& operator=(const E& e) {
+        descriptor dv, doff;
+        xg_expr xv = flatten_expr(as_expr(e), dv);
+        offsets(doff);
+        xg_expr xo = doff.expr();
+        xg_tensor t = source.tensor();
+        check(xg_assign_indexed(&xv, &xo, &t, current_stream()));
+        return *this;
+    }
+    xdev_index_view& operator=(const xdev_index_view& o) { return this->operator=<xdev_index_view>(o); }
+    template <class E> xdev_index_view& operator+=(const E& e) { return *this = *this + e; }
+    template <class E> xdev_index_view& operator-=(const E& e) { return *this = *this - e; }
+    template <class E> xdev_index_view& operator*=(const E& e) { return *this = *this * e; }
+    template <class E> xdev_index_view& operator/=(const E& e) { return *this = *this / e; }
+    void fill(T v) { *this = v; }
+};
+
+template <class T> xdev_index_view<T> index_view(const xdevice_array<T>& e, const xdevice_array<int64_t>& indices) {
+    return xdev_index_view<T>(e, indices);
+}
+template <class T>
+xdev_index_view<T> index_view(const xdevice_array<T>& e, const std::vector<std::vector<int64_t>>& indices) {
+    std::vector<int64_t> flat;
+    for (const auto& ix : indices) {
+        if (ix.size() != e.dimension()) throw std::invalid_argument("index_view: index of the wrong rank");
+        flat.insert(flat.end(), ix.begin(), ix.end());
+    }
+    return xdev_index_view<T>(e, xdevice_array<int64_t>::from_host(flat, {int64_t(indices.size()), int64_t(e.dimension())}));
+}
+
+// xt::argwhere (row-major order) of a device condition: positions listed on the host
+template <class C> std::vector<std::vector<int64_t>> argwhere(const C& cond) {
+    auto c = eval<uint8_t>(cond);
+    const std::vector<uint8_t> h = c.to_host();
+    std::vector<std::vector<int64_t>> out;
+    std::vector<int64_t> idx(c.dimension(), 0);
+    for (size_t i = 0; i < h.size(); ++i) {
+        if (h[i]) out.push_back(idx);
+        for (int k = int(idx.size()) - 1; k >= 0; --k) {
+            if (++idx[size_t(k)] < c.shape()[size_t(k)]) break;
+            idx[size_t(k)] = 0;
+        }
+    }
+    return out;
+}
+
+// xt::filter(e, cond) = index_view(e, argwhere(cond)) (views/xindex_view.hpp:820-826)
+template <class T, class C> xdev_index_view<T> filter(const xdevice_array<T>& e, const C& cond) {
+    auto pos = argwhere(cond);
+    if (pos.empty()) return xdev_index_view<T>(e, xdevice_array<int64_t>({0, int64_t(e.dimension())}));
+    return index_view(e, pos);
+}
+
+// xt::filtration(e, cond) (views/xindex_view.hpp:845-850) and masked_view
+// assignment (views/xmasked_view.hpp): e = where(cond, value, e), one launch
+template <class T, class C> struct xdev_filtration {
+    xdevice_array<T> e;
+    C cond;
+    template <class V> xdev_filtration& operator=(const V& v) {
+        noalias(e) = where(cond, as_expr(v), e);
+        return *this;
+    }
+    template <class V> xdev_filtration& operator+=(const V& v) { noalias(e) = where(cond, e + v, e); return *this; }
+    template <class V> xdev_filtration& operator-=(const V& v) { noalias(e) = where(cond, e - v, e); return *this; }
+    template <class V> xdev_filtration& operator*=(const V& v) { noalias(e) = where(cond, e * v, e); return *this; }
+    template <class V> xdev_filtration& operator/=(const V& v) { noalias(e) = where(cond, e / v, e); return *this; }
+};
+template <class T, class C> xdev_filtration<T, C> filtration(const xdevice_array<T>& e, const C& cond) {
+    return xdev_filtration<T, C>{e, cond};
+}
+template <class T, class C> xdev_filtration<T, C> masked_view(const xdevice_array<T>& e, const C& mask) {
+    return xdev_filtration<T, C>{e, mask};
+}
 
 // ---------------------------------------------------------------- .npy files
 // xt::load_npy<T>(path) (io/xnpy.hpp:775-800): the file's data in a new device
