@@ -102,13 +102,13 @@ void* workspace(size_t bytes, hipStream_t stream) {
 using namespace xg;
 
 // plan with the compiled-kernel vector width; without a matching compiled
-// program, re-plan for the generic interpreter (V = 4)
+// program, re-plan for the generic interpreter (V = 1, registers in LDS)
 EwPlan xg::plan_assign_best(const xg_expr* e, const xg_tensor* out, int* V) {
     *V = choose_vector_width(e, out);
     EwPlan p = plan_assign(e, out, *V);
-    if (*V != 4 && !ew_find_spec(p, *V) && !ew_jit_ok(p, *V)) {
-        *V = 4;
-        p = plan_assign(e, out, 4);
+    if (!ew_find_spec(p, *V) && !ew_jit_ok(p, *V)) {  // the generic interpreter, one element per item
+        *V = 1;
+        p = plan_assign(e, out, 1);
     }
     return p;
 }

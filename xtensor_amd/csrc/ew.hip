@@ -65,6 +65,56 @@ __global__ __launch_bounds__(256) void ew_kernel(const EwPlan plan) {
     }
 }
 
+// The generic interpreter for elementwise plans of vector width 1: one
+// element per work item and the program's register file in LDS, indexed
+// [register][thread] (a register array indexed by run-time operands would
+// live in scratch memory).  Each micro-op is one out-of-line call.
+template <bool I64>
+__global__ __launch_bounds__(256) void ew_lds_kernel(const EwPlan plan_arg) {
+    // the plan is read where the launch put it (the kernel-argument segment):
+    // indexing the by-value argument with run-time operands would copy the
+    // whole plan into scratch memory
+    (void)plan_arg;
+    const EwPlan& plan = *(const EwPlan*)__builtin_amdgcn_kernarg_segment_ptr();
+    __shared__ Reg regs[kMaxRegs][256];
+    const int tid = int(threadIdx.x);
+    const uint32_t nunits = uint32_t(plan.units);
+    const uint32_t step = gridDim.x * 256u;
+    for (uint32_t u = blockIdx.x * 256u + uint32_t(tid); u < nunits; u += step) {
+        const uint32_t row = fastdiv(u, plan.div_chunks);
+        const uint32_t j0 = u - row * plan.nchunks;
+        uint32_t idx[kMaxRank];
+        uint32_t r = row;
+#pragma unroll
+        for (int k = kMaxRank - 2; k >= 0; --k) {
+            if (k <= plan.rank - 2) {
+                const uint32_t q = fastdiv(r, plan.div[k]);
+                idx[k] = r - q * uint32_t(plan.shape[k]);
+                r = q;
+            } else {
+                idx[k] = 0;
+            }
+        }
+        for (int pc = 0; pc < plan.prog.nops; ++pc) {
+            const MicroOp m = plan.prog.ops[pc];
+            Reg v[1];
+            if (m.op == MO_LOAD) {
+                const LeafView& lv = plan.leaves[m.aux];
+                load_leaf<1, I64>(lv, lv.mode, lv.dtype, idx, plan.rank, j0, 1, v);
+            } else if (m.op == MO_GATHER) {
+                const LeafView& lv = plan.leaves[m.aux];
+                const Reg off[1] = {regs[m.a][tid]};
+                gather_vals<1>(lv.ptr, lv.dtype, lv.stride[0], lv.stride[1], 1, off, v);
+            } else {
+                v[0] = exec_scalar_dyn(m, regs[m.a][tid], regs[m.b][tid], regs[m.c][tid], plan.prog.consts);
+            }
+            regs[m.dst][tid] = v[0];
+        }
+        const Reg res[1] = {regs[plan.prog.result_reg][tid]};
+        store_out<1, I64>(plan, lm_base(plan.out_mode), plan.out_dtype, idx, j0, 1, res);
+    }
+}
+
 }  // namespace xg
 
 // compiled specialisations (generated from specs_list.cpp at build time)
@@ -220,10 +270,16 @@ hipError_t launch_ew(const EwPlan& p, int V, int cus, hipStream_t stream, const 
         if (used) *used = k->name.c_str();
         return jit_launch(k, grid.x, 256, &p, sizeof(EwPlan), stream);
     }
-    if (V != 4) return hipErrorInvalidValue;  // the generic interpreter is built for V = 4
+    if (V != 1 && V != 4) return hipErrorInvalidValue;  // the generic interpreters: V = 1 (LDS) or 4
     if (used) *used = "dynamic";
-    if (p.index64) hipLaunchKernelGGL((ew_kernel<4, kMaxRegs, DynSpec, true>), grid, block, 0, stream, p);
-    else hipLaunchKernelGGL((ew_kernel<4, kMaxRegs, DynSpec, false>), grid, block, 0, stream, p);
+    if (V == 1) {
+        if (p.index64) hipLaunchKernelGGL((ew_lds_kernel<true>), grid, block, 0, stream, p);
+        else hipLaunchKernelGGL((ew_lds_kernel<false>), grid, block, 0, stream, p);
+    } else if (p.index64) {
+        hipLaunchKernelGGL((ew_kernel<4, kMaxRegs, DynSpec, true>), grid, block, 0, stream, p);
+    } else {
+        hipLaunchKernelGGL((ew_kernel<4, kMaxRegs, DynSpec, false>), grid, block, 0, stream, p);
+    }
     return hipGetLastError();
 }
 

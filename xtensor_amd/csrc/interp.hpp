@@ -279,7 +279,9 @@ __device__ __forceinline__ uint64_t load_raw(const char* p, int esz) {
 template <int V, bool NT = false>
 __device__ __forceinline__ void load_vec(const char* p, int esz, int dt, Reg (&out)[V]) {
     const int bytes = V * esz;
-    if (bytes >= 16) {
+    if constexpr (V == 1) {
+        out[0] = from_storage(load_raw<NT>(p, esz), dt);
+    } else if (bytes >= 16) {
         constexpr int MAXW = V * 2;  // up to 8-byte elements
         uint32_t w[MAXW];
 #pragma unroll
@@ -359,7 +361,11 @@ __device__ __forceinline__ void store_raw(char* p, int esz, uint64_t raw) {
 template <int V, bool NT = false>
 __device__ __forceinline__ void store_vec(char* p, int esz, int dt, const Reg (&in)[V]) {
     const int bytes = V * esz;
-    if (bytes >= 16) {
+    // one element per lane: a plain store (the packing below masks each
+    // element to esz bytes, which needs esz < bytes)
+    if constexpr (V == 1) {
+        store_raw<NT>(p, esz, to_storage(in[0], dt));
+    } else if (bytes >= 16) {
         constexpr int MAXW = V * 2;
         uint32_t w[MAXW];
 #pragma unroll
