@@ -78,6 +78,19 @@ __device__ __forceinline__ void run_static(const LeafView* leaves, const uint64_
     }
 }
 
+// A kernel's by-value plan argument, read where the launch put it (the
+// kernel-argument segment; the first argument sits at offset 0).  Indexing
+// a by-value struct argument with run-time operands (leaves[m.aux],
+// ops[pc]) makes the compiler copy the whole struct into scratch memory
+// first, ~3 KB per lane for these plans; reads through the segment pointer
+// stay scalar loads.  Kernels instantiated on compiled programs index the
+// plan with constants only and keep the argument itself.
+template <class S, class P>
+__device__ __forceinline__ const P& kernel_plan(const P& arg) {
+    if constexpr (S::is_static) return arg;
+    else return *(const P*)__builtin_amdgcn_kernarg_segment_ptr();
+}
+
 // The dynamic interpreter calls one out-of-line function per micro-op so
 // that kernels instantiated on it stay small.
 static __device__ __noinline__ Reg exec_scalar_dyn(MicroOp m, Reg a, Reg b, Reg c, const uint64_t* kconst) {

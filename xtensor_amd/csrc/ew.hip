@@ -38,44 +38,13 @@ __device__ __forceinline__ void store_out(const EwPlan& plan, int mode, int dt, 
     }
 }
 
-template <int V, int NR, class S, bool I64>
-__global__ __launch_bounds__(256) void ew_kernel(const EwPlan plan) {
-    const uint32_t nunits = uint32_t(plan.units);
-    const uint32_t step = gridDim.x * 256u;
-    for (uint32_t u = blockIdx.x * 256u + threadIdx.x; u < nunits; u += step) {
-        const uint32_t row = fastdiv(u, plan.div_chunks);
-        const uint32_t j0 = (u - row * plan.nchunks) * V;
-        uint32_t idx[kMaxRank];
-        uint32_t r = row;
-#pragma unroll
-        for (int k = kMaxRank - 2; k >= 0; --k) {
-            if (k <= plan.rank - 2) {
-                uint32_t q = fastdiv(r, plan.div[k]);
-                idx[k] = r - q * uint32_t(plan.shape[k]);
-                r = q;
-            } else {
-                idx[k] = 0;
-            }
-        }
-        const int nvalid = min(V, int(plan.inner - j0));
-        Reg res[V];
-        eval_chunk<V, NR, S, I64>(plan.leaves, plan.prog, plan.rank, idx, j0, nvalid, res);
-        if constexpr (S::is_static) store_out<V, I64>(plan, S::out_mode, S::out_dtype, idx, j0, nvalid, res);
-        else store_out<V, I64>(plan, lm_base(plan.out_mode), plan.out_dtype, idx, j0, nvalid, res);
-    }
-}
-
 // The generic interpreter for elementwise plans of vector width 1: one
 // element per work item and the program's register file in LDS, indexed
 // [register][thread] (a register array indexed by run-time operands would
 // live in scratch memory).  Each micro-op is one out-of-line call.
 template <bool I64>
 __global__ __launch_bounds__(256) void ew_lds_kernel(const EwPlan plan_arg) {
-    // the plan is read where the launch put it (the kernel-argument segment):
-    // indexing the by-value argument with run-time operands would copy the
-    // whole plan into scratch memory
-    (void)plan_arg;
-    const EwPlan& plan = *(const EwPlan*)__builtin_amdgcn_kernarg_segment_ptr();
+    const EwPlan& plan = kernel_plan<DynSpec>(plan_arg);
     __shared__ Reg regs[kMaxRegs][256];
     const int tid = int(threadIdx.x);
     const uint32_t nunits = uint32_t(plan.units);
@@ -270,16 +239,10 @@ hipError_t launch_ew(const EwPlan& p, int V, int cus, hipStream_t stream, const 
         if (used) *used = k->name.c_str();
         return jit_launch(k, grid.x, 256, &p, sizeof(EwPlan), stream);
     }
-    if (V != 1 && V != 4) return hipErrorInvalidValue;  // the generic interpreters: V = 1 (LDS) or 4
+    if (V != 1) return hipErrorInvalidValue;  // the generic interpreter takes one element per lane
     if (used) *used = "dynamic";
-    if (V == 1) {
-        if (p.index64) hipLaunchKernelGGL((ew_lds_kernel<true>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((ew_lds_kernel<false>), grid, block, 0, stream, p);
-    } else if (p.index64) {
-        hipLaunchKernelGGL((ew_kernel<4, kMaxRegs, DynSpec, true>), grid, block, 0, stream, p);
-    } else {
-        hipLaunchKernelGGL((ew_kernel<4, kMaxRegs, DynSpec, false>), grid, block, 0, stream, p);
-    }
+    if (p.index64) hipLaunchKernelGGL((ew_lds_kernel<true>), grid, block, 0, stream, p);
+    else hipLaunchKernelGGL((ew_lds_kernel<false>), grid, block, 0, stream, p);
     return hipGetLastError();
 }
 

@@ -102,7 +102,8 @@ __device__ A group_combine(A a, int G, Store* lds) {
 
 // ---------------------------------------------------------------- rows layout
 template <int V, int NR, class S, bool I64, int AK, int OP, int kUnroll>
-__global__ __launch_bounds__(256) void reduce_rows_kernel(const RedPlan p) {
+__global__ __launch_bounds__(256) void reduce_rows_kernel(const RedPlan p_arg) {
+    const RedPlan& p = kernel_plan<S>(p_arg);
     using A = Acc<AK, OP>;
     __shared__ Store lds[4];
     const int G = p.group;
@@ -169,7 +170,8 @@ __global__ __launch_bounds__(256) void reduce_rows_kernel(const RedPlan p) {
 
 // ---------------------------------------------------------------- cols layout
 template <int V, int NR, class S, bool I64, int AK, int OP, int kUnroll>
-__global__ __launch_bounds__(256) void reduce_cols_kernel(const RedPlan p) {
+__global__ __launch_bounds__(256) void reduce_cols_kernel(const RedPlan p_arg) {
+    const RedPlan& p = kernel_plan<S>(p_arg);
     using A = Acc<AK, OP>;
     __shared__ Store lds[4][64][V];
     const int lane = threadIdx.x & 63;

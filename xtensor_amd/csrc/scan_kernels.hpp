@@ -39,7 +39,8 @@ __device__ __forceinline__ void scan_store(const ScanPlan& p, const uint32_t* id
 }
 
 template <int V, int NR, class S, bool I64, int AK, int OP>
-__global__ __launch_bounds__(256) void scan_rows_kernel(const ScanPlan p) {
+__global__ __launch_bounds__(256) void scan_rows_kernel(const ScanPlan p_arg) {
+    const ScanPlan& p = kernel_plan<S>(p_arg);
     using A = Acc<AK, OP>;
     __shared__ Store wtot[4];
     const int lane = threadIdx.x & 63;
@@ -109,7 +110,8 @@ __global__ __launch_bounds__(256) void scan_rows_kernel(const ScanPlan p) {
 }
 
 template <int V, int NR, class S, bool I64, int AK, int OP>
-__global__ __launch_bounds__(256) void scan_cols_kernel(const ScanPlan p) {
+__global__ __launch_bounds__(256) void scan_cols_kernel(const ScanPlan p_arg) {
+    const ScanPlan& p = kernel_plan<S>(p_arg);
     using A = Acc<AK, OP>;
     for (uint32_t item = blockIdx.x * 256 + threadIdx.x; item < p.rows; item += gridDim.x * 256) {
         uint32_t oo = fastdiv(item, p.div_nchunks_in);
