@@ -82,6 +82,36 @@ def test_f32_cumsum_rows(rows, cols):
     assert err <= 2e-7, err
 
 
+@pytest.mark.parametrize("dtype", ["int32", "int64", "float32", "float64"])
+def test_direct_rows_scan_lengths(dtype):
+    """Contiguous rows scans around the one-wave / four-wave tile edges
+    (1024 elements of 4-byte types, 512 of 8-byte types per wave tile, four
+    wave tiles per block tile), sums and products: integers exact, floats
+    against the oracle's long-double scan (rel 2e-7 for float32, 1e-14 for
+    float64)."""
+    rng = np.random.default_rng(11)
+    lengths = [1, 3, 511, 512, 513, 1023, 1024, 1025, 2048, 4095, 4096, 4097, 9000]
+    for n in lengths:
+        rows = 3
+        if dtype.startswith("int"):
+            x = rng.integers(-50, 50, (rows, n)).astype(dtype)
+        else:
+            x = rng.normal(0.5, 1, (rows, n)).astype(dtype)
+        X = Dual.of(x)
+        o = oracle_for(X)
+        got = xt.cumsum(X.dev, 1).to_numpy()
+        if dtype.startswith("int"):
+            assert np.array_equal(got, o.scan(X.dev, 1)), n
+            p = rng.choice(np.array([-1, 1, 1, 1], dtype=dtype), (rows, n))
+            P = Dual.of(p)
+            gp = xt.cumprod(P.dev, 1).to_numpy()
+            assert np.array_equal(gp, np.cumprod(p, axis=1)), n
+        else:
+            ref = o.scan(X.dev, 1, high_precision=True)
+            err = np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0))
+            assert err <= (2e-7 if dtype == "float32" else 1e-14), (n, err)
+
+
 def test_scan_strided_axis_and_ints():
     rng = np.random.default_rng(3)
     x = rng.integers(-1000, 1000, (300, 7, 33), dtype=np.int32)

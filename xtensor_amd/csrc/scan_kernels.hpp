@@ -162,36 +162,98 @@ template <> struct ScanIn<AK_U32> { using T = uint32_t; };
 template <> struct ScanIn<AK_I64> { using T = int64_t; };
 template <> struct ScanIn<AK_U64> { using T = uint64_t; };
 
+// Cross-lane moves of an accumulator through DPP (the data moves between
+// lanes inside a VALU instruction, no LDS permute), one 32-bit word at a
+// time.  Lanes the pattern gives no source, or that ROW_MASK leaves out,
+// take `old`.  dpp_ctrl codes of GFX9 DPP: row_shr:n = 0x110 + n,
+// wave_shr:1 = 0x138, row_bcast:15 = 0x142, row_bcast:31 = 0x143.
+template <int CTRL, int ROW_MASK, class A>
+__device__ __forceinline__ A dpp_move(const A& src, const A& old) {
+    static_assert(sizeof(A) % 4 == 0, "accumulators are whole 32-bit words");
+    constexpr int N = int(sizeof(A) / 4);
+    int s[N], o[N], r[N];
+    __builtin_memcpy(s, &src, sizeof(A));
+    __builtin_memcpy(o, &old, sizeof(A));
+#pragma unroll
+    for (int i = 0; i < N; ++i) r[i] = __builtin_amdgcn_update_dpp(o[i], s[i], CTRL, ROW_MASK, 0xf, false);
+    A out;
+    __builtin_memcpy(&out, r, sizeof(A));
+    return out;
+}
+
+// the accumulator of lane `L`, in every lane
+template <int L, class A>
+__device__ __forceinline__ A lane_value(const A& src) {
+    constexpr int N = int(sizeof(A) / 4);
+    int s[N];
+    __builtin_memcpy(s, &src, sizeof(A));
+#pragma unroll
+    for (int i = 0; i < N; ++i) s[i] = __builtin_amdgcn_readlane(s[i], L);
+    A out;
+    __builtin_memcpy(&out, s, sizeof(A));
+    return out;
+}
+
+// Inclusive scan of one accumulator per lane over the wave, earlier lanes
+// merged first: shifts of 1, 2, 4, 8 inside each row of 16 lanes, then row
+// 0's total into row 1 and row 2's into row 3, then the total of rows 0-1
+// into rows 2 and 3.  `id` is the identity.
+template <class A>
+__device__ __forceinline__ A wave_inclusive_scan(A x, const A& id) {
+    { A y = dpp_move<0x111, 0xf>(x, id); y.merge(x); x = y; }
+    { A y = dpp_move<0x112, 0xf>(x, id); y.merge(x); x = y; }
+    { A y = dpp_move<0x114, 0xf>(x, id); y.merge(x); x = y; }
+    { A y = dpp_move<0x118, 0xf>(x, id); y.merge(x); x = y; }
+    { A y = dpp_move<0x142, 0xa>(x, id); y.merge(x); x = y; }
+    { A y = dpp_move<0x143, 0xc>(x, id); y.merge(x); x = y; }
+    return x;
+}
+
 // Direct rows scan for a program that only loads its one leaf (already of
 // the accumulation input type), contiguous and 16-byte aligned along the
-// axis, with a contiguous aligned output: one wave per row (rank 2,
-// [rows | axis]).  The row goes by in segments of 64 * V elements, U of
-// them loaded at once; per segment each lane scans its V elements, a wave
-// scan (shuffles, earlier lanes first) gives the exclusive prefix, and the
-// segment total carries on.  The merge order depends on the row length
-// only.
-template <int AK, int OP>
+// axis, with a contiguous aligned output (rank 2, [rows | axis]).  W waves
+// share a row: W = 1 gives one row per wave (four rows per block), for rows
+// that fit one wave's tile; W = 4 gives one row per block.  A tile of the
+// row is W * U segments of 64 * V elements (16 bytes per lane, so a wave
+// instruction covers 1 KiB); wave w owns segments [w*U, (w+1)*U) and loads
+// them at once.  Per segment each lane sums its V elements, a wave scan
+// (DPP moves, earlier lanes first) gives the exclusive prefix and the
+// segment total carries on inside the wave; the W wave totals meet in LDS,
+// so every wave knows the tile's carry plus the waves before it, and each
+// lane then runs its V outputs up from that prefix.  The merge order
+// depends on the row length only.  NATIVE: the result and the output are
+// of the input's type (the usual cumsum / cumprod), fixed at compile time
+// so that no per-element type conversion is dispatched at run time.
+template <int AK, int OP, bool NATIVE, int W>
 __global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p) {
     using A = Acc<AK, OP>;
     using T = typename ScanIn<AK>::T;
-    constexpr int V = sizeof(T) == 4 ? 4 : 2;
+    constexpr int V = int(16 / sizeof(T));
     constexpr int U = 4;
-    constexpr uint32_t BLK = 64u * V * U;  // elements of one block of U segments
+    constexpr uint32_t SEG = 64u * V;
+    constexpr uint32_t TILE = SEG * U * W;
+    __shared__ Store wtot[4];
     const int lane = threadIdx.x & 63;
-    const uint32_t row = blockIdx.x * 4u + uint32_t(threadIdx.x >> 6);
-    if (row >= p.rows) return;  // whole waves leave: row is uniform in a wave
+    const int wave = threadIdx.x >> 6;
+    const uint32_t row = W == 1 ? blockIdx.x * 4u + uint32_t(wave) : blockIdx.x;
+    const int wr = W == 1 ? 0 : wave;  // this wave's place in a tile
+    if (row >= p.rows) return;  // uniform in a wave (W = 1) or in the block (W = 4)
     const T* in = reinterpret_cast<const T*>(p.leaves[0].ptr) + int64_t(row) * p.leaves[0].stride[0];
     const bool nt_in = lm_nt(p.leaves[0].mode), nt_out = lm_nt(p.out_mode);
-    const int odt = p.out_dtype;
+    const int odt = NATIVE ? int(ScanInDtype<T>::value) : p.out_dtype;
+    const int rdt = NATIVE ? int(ScanInDtype<T>::value) : p.rdt;
     const int oesz = dtype_size(odt);
     char* out = p.out_ptr + int64_t(row) * p.out_stride[0] * oesz;
-    A carry;
-    carry.init(p.identity);
-    // load block `base` (U segments of 64 * V elements) into x / nv
-    auto load = [&](uint32_t base, T (&x)[U][V], int (&nv)[U]) {
+    A id;
+    id.init(p.identity);
+    A carry = id;  // everything before the current tile
+    for (uint32_t base = 0; base < p.len; base += TILE) {
+        const uint32_t wbase = base + uint32_t(wr) * SEG * U;
+        T x[U][V];
+        int nv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const uint32_t j = base + uint32_t(u * 64 * V) + uint32_t(lane * V);
+            const uint32_t j = wbase + uint32_t(u) * SEG + uint32_t(lane * V);
             nv[u] = j < p.len ? min(V, int(p.len - j)) : 0;
             if (nv[u] == V) {
                 Reg r[V];
@@ -204,41 +266,50 @@ __global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p)
                 for (int v = 0; v < V; ++v) x[u][v] = v < nv[u] ? in[j + uint32_t(v)] : T(0);
             }
         }
-    };
-    // scan block `base` from x / nv, store it, carry on
-    auto process = [&](uint32_t base, const T (&x)[U][V], const int (&nv)[U]) {
+        // wave-local exclusive prefixes of the segments
+        A local = id;
+        A pre[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (base + uint32_t(u * 64 * V) >= p.len) break;  // uniform
-            A run[V];
-            A t;
-            t.init(p.identity);
+            pre[u] = local;
+            if (wbase + uint32_t(u) * SEG >= p.len) continue;  // uniform in the wave
+            A t = id;
 #pragma unroll
-            for (int v = 0; v < V; ++v) {
+            for (int v = 0; v < V; ++v)
                 if (v < nv[u]) t.add(reg_put<T>(x[u][v]));
-                run[v] = t;
+            const A incl = wave_inclusive_scan(t, id);
+            pre[u].merge(dpp_move<0x138, 0xf>(incl, id));  // wave_shr:1, lane 0 takes the identity
+            local.merge(lane_value<63>(incl));
+        }
+        A before = carry;  // the tile's carry and the waves before this one
+        if constexpr (W > 1) {
+            if (lane == 0) wtot[wave] = local.store();
+            __syncthreads();
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                if (w == wave) before = carry;
+                A o;
+                o.load(wtot[w]);
+                carry.merge(o);
             }
-            A incl = t;
-            for (int d = 1; d < 64; d <<= 1) {
-                A y = incl.shfl_up(d);
-                if (lane >= d) {
-                    y.merge(incl);
-                    incl = y;
-                }
-            }
-            A ex = incl.shfl_up(1);
-            if (lane == 0) ex.init(p.identity);
-            A pre = carry;
-            pre.merge(ex);
+            __syncthreads();  // wtot is written again for the next tile
+        } else {
+            carry.merge(local);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (wbase + uint32_t(u) * SEG >= p.len) continue;  // uniform in the wave
+            A t = before;
+            t.merge(pre[u]);
             Reg o[V];
 #pragma unroll
             for (int v = 0; v < V; ++v) {
-                A q = pre;
-                q.merge(run[v]);
-                const Reg val = q.value(p.rdt);
-                o[v] = odt == XG_BFLOAT16 ? cvt(val, p.rdt, XG_FLOAT32) : cvt(val, p.rdt, odt);
+                if (v < nv[u]) t.add(reg_put<T>(x[u][v]));
+                const Reg val = t.value(rdt);
+                if constexpr (NATIVE) o[v] = val;
+                else o[v] = odt == XG_BFLOAT16 ? cvt(val, rdt, XG_FLOAT32) : cvt(val, rdt, odt);
             }
-            const uint32_t j = base + uint32_t(u * 64 * V) + uint32_t(lane * V);
+            const uint32_t j = wbase + uint32_t(u) * SEG + uint32_t(lane * V);
             char* ptr = out + int64_t(j) * oesz;
             if (nv[u] == V) {
                 if (nt_out) store_vec<V, true>(ptr, oesz, odt, o);
@@ -248,23 +319,20 @@ __global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p)
                 for (int v = 0; v < V; ++v)
                     if (v < nv[u]) store_raw(ptr + int64_t(v) * oesz, oesz, to_storage(o[v], odt));
             }
-            carry.merge(incl.shfl_idx(63));
         }
-    };
-    // one register set (a second one, to overlap the next block's loads with
-    // this block's scan, measured slower: 5 instead of 8 waves per SIMD)
-    for (uint32_t base = 0; base < p.len; base += BLK) {
-        T x[U][V];
-        int nv[U];
-        load(base, x, nv);
-        process(base, x, nv);
     }
 }
+
+// elements of one wave's tile in the direct rows scan (rows up to this
+// length take one wave each)
+template <class T> constexpr uint32_t scan_wave_tile() { return 64u * uint32_t(16 / sizeof(T)) * 4u; }
 
 using SFn = void (*)(const ScanPlan);
 
 struct ScanKernels {
-    SFn rows32, rows64, cols32, cols64, rows_direct;
+    SFn rows32, rows64, cols32, cols64;
+    SFn direct[2][2];   // [native][waves per row: 1, 4]
+    uint32_t wave_tile;  // elements of one wave's tile in the direct kernels
 };
 
 template <int AK, int OP> ScanKernels scan_kernels() {
@@ -272,7 +340,9 @@ template <int AK, int OP> ScanKernels scan_kernels() {
                        scan_rows_kernel<4, kMaxRegs, DynSpec, true, AK, OP>,
                        scan_cols_kernel<4, kMaxRegs, DynSpec, false, AK, OP>,
                        scan_cols_kernel<4, kMaxRegs, DynSpec, true, AK, OP>,
-                       scan_rows_direct_kernel<AK, OP>};
+                       {{scan_rows_direct_kernel<AK, OP, false, 1>, scan_rows_direct_kernel<AK, OP, false, 4>},
+                        {scan_rows_direct_kernel<AK, OP, true, 1>, scan_rows_direct_kernel<AK, OP, true, 4>}},
+                       scan_wave_tile<typename ScanIn<AK>::T>()};
 }
 
 bool scan_kernels_float(int ak, int op, ScanKernels* k);
