@@ -158,10 +158,12 @@ hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** use
             if (blocks >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
             err = go(q.rank == 1 ? ks.rows_fast1 : ks.rows_fast2, uint32_t(blocks), 256, q);
         } else {
-            const uint64_t tiles = (q.nchunks_f + 63) / 64;
+            const uint64_t tw = uint64_t(q.group);  // tile width in lanes
+            if (tw != 64 && tw != 128 && tw != 256) return hipErrorInvalidValue;
+            const uint64_t tiles = (q.nchunks_f + tw - 1) / tw;
             const uint64_t blocks = tiles * uint64_t(q.splits);
             if (blocks >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
-            err = go(ks.cols_fast, uint32_t(blocks), 64 * kColsWaves, q);
+            err = go(ks.cols_fast, uint32_t(blocks), 512, q);
         }
         if (err != hipSuccess || q.splits == 1) return err;
         return launch_combine(q, stream);

@@ -39,10 +39,6 @@ enum Finalize : int32_t { FZ_NONE = 0, FZ_MEAN = 1, FZ_SQRT = 2, FZ_MEAN_SQRT = 
 constexpr int kRedMinNum = 100;
 constexpr int kRedMaxNum = 101;
 
-// waves per workgroup of the compiled cols kernel (they split the rows of
-// one column tile and combine in LDS)
-constexpr int kColsWaves = 8;
-
 struct RedPlan {
     int32_t layout;
     int32_t rank;       // normalised dims (see below)
@@ -86,7 +82,7 @@ struct RedPlan {
     // 32-bit offsets (rows: [outputs | fast] or [fast]; cols: [rows | fast]).
     // cst[l][k] = stride of leaf l along dim k.  f_* = their work split.
     int32_t compact;
-    int32_t f_group;      // rows: lanes per output (1..64, power of two)
+    int32_t f_group;      // rows: lanes per output (1..64, power of two); cols: tile width in lanes (64, 128, 256)
     int32_t f_splits;     // partial results per output
     uint32_t f_per_split; // rows: chunks per split; cols: rows per split
     int32_t cst[kMaxLeaves + 1][4];

@@ -309,17 +309,31 @@ __global__ __launch_bounds__(256) void reduce_combine_kernel(const RedPlan p) {
 }
 
 // a block per output, for many partials: thread t merges partials
-// t, t + 256, ... in order, then the block combines in a fixed tree
+// t, t + 256, ... in order (loaded 16 at a time, so a few thousand partials
+// cost one or two memory round trips), then the block combines in a fixed
+// tree
 template <int AK, int OP>
 __global__ __launch_bounds__(256) void reduce_combine_wide_kernel(const RedPlan p) {
     using A = Acc<AK, OP>;
+    constexpr int B = 16;
     __shared__ Store lds[4];
     for (uint32_t o = blockIdx.x; o < p.n_out; o += gridDim.x) {
         const Store* part = reinterpret_cast<const Store*>(p.partials) + size_t(o) * p.splits;
         A a;
         a.init(p.identity);
-        if (int(threadIdx.x) < p.splits) a.load(part[threadIdx.x]);
-        for (int s = int(threadIdx.x) + 256; s < p.splits; s += 256) {
+        int s = int(threadIdx.x);
+        for (; s + (B - 1) * 256 < p.splits; s += B * 256) {
+            Store st[B];
+#pragma unroll
+            for (int k = 0; k < B; ++k) st[k] = part[s + k * 256];
+#pragma unroll
+            for (int k = 0; k < B; ++k) {
+                A b;
+                b.load(st[k]);
+                a.merge(b);
+            }
+        }
+        for (; s < p.splits; s += 256) {
             A b;
             b.load(part[s]);
             a.merge(b);
@@ -336,10 +350,25 @@ __global__ __launch_bounds__(256) void reduce_combine_wide_kernel(const RedPlan 
 // (no LDS, no barrier).  p.group / p.splits / p.per_split hold the fast
 // split (f_group / f_splits / f_per_split of the plan).
 
-// chunks (rows) in flight per lane: the rows kernel keeps more waves
-// resident with 4, the cols kernel gains from 8 on one-register programs
+// the non-temporal flags of a compiled program's leaves that are not
+// broadcast: 1 if all are set, 0 if none is, 2 if they differ
+template <class S>
+__device__ __forceinline__ int spec_leaves_nt(const RedPlan& p) {
+    bool all = true, none = true;
+#pragma unroll
+    for (int l = 0; l < S::nleaves; ++l) {
+        if (lm_base(S::leaf_mode[l]) == LM_BCAST) continue;
+        const bool nt = lm_nt(p.leaves[l].mode);
+        all = all && nt;
+        none = none && !nt;
+    }
+    return all ? 1 : none ? 0 : 2;
+}
+
+// chunks (rows) in flight per lane (the cols kernel measured 4.2 TB/s with
+// 4 and 4.0 with 8 on a [8192, 4096] float32 column sum)
 template <int NR, int V, bool COLS> constexpr int fast_unroll() {
-    return (COLS && NR * V <= 4) ? 8 : (NR * V <= 8 ? 4 : (NR * V <= 16 ? 2 : 1));
+    return NR * V <= 8 ? 4 : (NR * V <= 16 ? 2 : 1);
 }
 
 // rows: dims [outputs | fast] (RANK 2) or [fast] (RANK 1, one output);
@@ -356,92 +385,139 @@ __global__ __launch_bounds__(256) void reduce_rows_fast_kernel(const RedPlan p) 
     const uint32_t items = p.n_out * uint32_t(p.splits);
     A acc;
     acc.init(p.identity);
-    uint32_t o = 0;
-    if (wi < items) {
-        o = fastdiv(wi, p.div_splits);
-        const uint32_t sp = wi - o * uint32_t(p.splits);
-        const uint32_t c0 = sp * p.per_split;
-        const uint32_t c1 = min(p.nchunks_f, c0 + p.per_split);
-        const uint32_t idx[4] = {RANK == 2 ? o : 0u, 0u, 0u, 0u};
+    const bool live = wi < items;
+    const uint32_t o = live ? fastdiv(wi, p.div_splits) : 0u;
+    const uint32_t sp = wi - o * uint32_t(p.splits);
+    const uint32_t c0 = sp * p.per_split;
+    const uint32_t c1 = live ? min(p.nchunks_f, c0 + p.per_split) : 0u;
+    const uint32_t idx[4] = {RANK == 2 ? o : 0u, 0u, 0u, 0u};
+    // FULL: every chunk of the lane's range is whole, so chunks past the
+    // range's end are clamped to its last chunk and their values dropped,
+    // and no load sits behind a divergent branch (U loads stay in flight)
+    auto sweep = [&](auto full_c, auto ntm_c) {
+        constexpr bool FULL = decltype(full_c)::value;
+        constexpr int NTM = decltype(ntm_c)::value;
         for (uint32_t c = c0 + gl; c < c1; c += uint32_t(G) * U) {
             Reg R[U][NR][V];
             int nv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t cc = c + uint32_t(u * G);
-                nv[u] = cc < c1 ? min(V, int(p.lf - cc * V)) : 0;
-                if (nv[u] > 0) run_spec<S, 0, V, NR, RANK>(p, idx, cc * V, nv[u], R[u]);
+                if constexpr (FULL) {
+                    nv[u] = cc < c1 ? V : 0;
+                    run_spec<S, 0, V, NR, RANK, true, NTM>(p, idx, min(cc, c1 - 1) * V, V, R[u]);
+                } else {
+                    nv[u] = cc < c1 ? min(V, int(p.lf - cc * V)) : 0;
+                    if (nv[u] > 0) run_spec<S, 0, V, NR, RANK>(p, idx, cc * V, nv[u], R[u]);
+                }
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u)
+            for (int u = 0; u < U; ++u) {
+                if constexpr (FULL) {  // a select, not a branch (see the cols kernel)
+                    A t = acc;
 #pragma unroll
-                for (int v = 0; v < V; ++v)
-                    if (v < nv[u]) acc.add(R[u][S::result_reg][v]);
+                    for (int v = 0; v < V; ++v) t.add(R[u][S::result_reg][v]);
+                    acc = nv[u] > 0 ? t : acc;
+                } else {
+#pragma unroll
+                    for (int v = 0; v < V; ++v)
+                        if (v < nv[u]) acc.add(R[u][S::result_reg][v]);
+                }
+            }
         }
-    }
+    };
+    // whole chunks up to the lane's range end (or an empty range): uniform over the wave
+    const bool full = __all(c1 <= c0 || c1 * uint32_t(V) <= p.lf);
+    const int ntm = spec_leaves_nt<S>(p);
+    if (full && ntm == 1) sweep(std::true_type{}, std::integral_constant<int, 1>{});
+    else if (full && ntm == 0) sweep(std::true_type{}, std::integral_constant<int, 0>{});
+    else sweep(std::false_type{}, std::integral_constant<int, 2>{});
     acc = wave_group_combine(acc, G);
-    if (wi < items && gl == 0) {
+    if (live && gl == 0) {
         if (p.splits == 1) finish(p, acc, o);
         else reinterpret_cast<Store*>(p.partials)[wi] = acc.store();
     }
 }
 
-// cols: dims [rows | fast outputs]; a block = one tile of 64 * V outputs and
-// one split of the rows, its kColsWaves waves interleaving rows, combined in
-// LDS in wave order
+// cols: dims [rows | fast outputs]; a block of 512 threads = one tile of
+// TW * V adjacent outputs (TW = p.group lanes: 64, 128 or 256) and one split
+// of the rows.  The block's 512 / TW row groups interleave rows, each lane
+// with U rows in flight, and combine in LDS in row-group order.
 template <int V, int NR, class S, int AK, int OP>
-__global__ __launch_bounds__(64 * kColsWaves) void reduce_cols_fast_kernel(const RedPlan p) {
+__global__ __launch_bounds__(512) void reduce_cols_fast_kernel(const RedPlan p) {
     using A = Acc<AK, OP>;
     constexpr int U = fast_unroll<NR, V, true>();
-    constexpr int W = kColsWaves;
-    __shared__ Store lds[W][64][V];
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    __shared__ Store lds[512][V];
+    const uint32_t TW = uint32_t(p.group);
+    const uint32_t RG = 512u / TW;
+    const uint32_t tl = threadIdx.x & (TW - 1);
+    const uint32_t rg = threadIdx.x / TW;
     // split fastest: the splits of one column strip run side by side (measured
     // faster than tile-fastest, which reads whole rows per group of blocks:
     // 3.6 vs 3.1 TB/s on a [8192, 4096] float32 column sum)
     const uint32_t tile = fastdiv(blockIdx.x, p.div_splits);
     const uint32_t sp = blockIdx.x - tile * uint32_t(p.splits);
-    const uint32_t cc = tile * 64 + uint32_t(lane);
+    const uint32_t cc = tile * TW + tl;
     const bool active = cc < p.nchunks_f;
     const uint32_t j0 = cc * V;
     const int nvalid = active ? min(V, int(p.lf - j0)) : 0;
     A acc[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v].init(p.identity);
-    if (active) {
-        const uint32_t r0 = sp * p.per_split;
-        const uint32_t r1 = min(p.red_count, r0 + p.per_split);
-        for (uint32_t r = r0 + uint32_t(wave); r < r1; r += W * U) {
+    const uint32_t r0 = sp * p.per_split;
+    const uint32_t r1 = min(p.red_count, r0 + p.per_split);
+    // the rows of this block's split; FULL: the whole tile holds whole
+    // chunks, so rows past the split's end are clamped to its last row and
+    // their values dropped, and no load sits behind a divergent branch
+    auto sweep = [&](auto full_c, auto ntm_c) {
+        constexpr bool FULL = decltype(full_c)::value;
+        constexpr int NTM = decltype(ntm_c)::value;
+        for (uint32_t r = r0 + rg; r < r1; r += RG * U) {
             Reg R[U][NR][V];
             bool ok[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint32_t rr = r + uint32_t(W * u);
+                const uint32_t rr = r + RG * uint32_t(u);
                 ok[u] = rr < r1;
-                const uint32_t idx[4] = {rr, 0u, 0u, 0u};
-                if (ok[u]) run_spec<S, 0, V, NR, 2>(p, idx, j0, nvalid, R[u]);
+                const uint32_t idx[4] = {FULL ? min(rr, r1 - 1) : rr, 0u, 0u, 0u};
+                if (FULL || ok[u]) run_spec<S, 0, V, NR, 2, FULL, NTM>(p, idx, j0, nvalid, R[u]);
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (ok[u])
+            for (int u = 0; u < U; ++u) {
+                if constexpr (FULL) {
+                    // a select, not a branch: a branch would let the compiler
+                    // sink the clamped row's load into it and wait there
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        A t = acc[v];
+                        t.add(R[u][S::result_reg][v]);
+                        acc[v] = ok[u] ? t : acc[v];
+                    }
+                } else if (ok[u]) {
 #pragma unroll
                     for (int v = 0; v < V; ++v)
                         if (v < nvalid) acc[v].add(R[u][S::result_reg][v]);
+                }
+            }
         }
-    }
+    };
+    const bool full_tile = (tile + 1) * TW <= p.nchunks_f && p.lf == p.nchunks_f * uint32_t(V);  // uniform
+    const int ntm = spec_leaves_nt<S>(p);
+    if (full_tile && ntm == 1) sweep(std::true_type{}, std::integral_constant<int, 1>{});
+    else if (full_tile && ntm == 0) sweep(std::true_type{}, std::integral_constant<int, 0>{});
+    else if (active) sweep(std::false_type{}, std::integral_constant<int, 2>{});
 #pragma unroll
-    for (int v = 0; v < V; ++v) lds[wave][lane][v] = acc[v].store();
+    for (int v = 0; v < V; ++v) lds[threadIdx.x][v] = acc[v].store();
     __syncthreads();
-    if (wave == 0 && active) {
+    if (rg == 0 && active) {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             if (v < nvalid) {
                 A a;
-                a.load(lds[0][lane][v]);
-                for (int w = 1; w < W; ++w) {
+                a.load(lds[tl][v]);
+                for (uint32_t g = 1; g < RG; ++g) {
                     A b;
-                    b.load(lds[w][lane][v]);
+                    b.load(lds[g * TW + tl][v]);
                     a.merge(b);
                 }
                 const uint32_t o = j0 + uint32_t(v);

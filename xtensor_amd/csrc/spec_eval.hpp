@@ -22,19 +22,22 @@ __device__ __forceinline__ int32_t coff(const int32_t (&st)[4], const uint32_t (
     return off;
 }
 
-template <int V, bool NT = false>
+// FULL: every lane's V elements are in range (no per-lane tail test, so the
+// loads of several chunks carry no divergent branch and stay in flight
+// together)
+template <int V, bool NT = false, bool FULL = false>
 __device__ __forceinline__ void load_at(const char* p, int mode, int dt, int32_t is, int nvalid, Reg (&out)[V]) {
     const int esz = dtype_size(dt);
     if (mode == LM_BCAST) {
         Reg x = from_storage(load_raw(p, esz), dt);
 #pragma unroll
         for (int v = 0; v < V; ++v) out[v] = x;
-    } else if (mode == LM_CONTIG && nvalid == V) {
+    } else if (mode == LM_CONTIG && (FULL || nvalid == V)) {
         load_vec<V, NT>(p, esz, dt, out);
     } else {
 #pragma unroll
         for (int v = 0; v < V; ++v)
-            out[v] = v < nvalid ? from_storage(load_raw<NT>(p + int64_t(v) * is * esz, esz), dt) : 0;
+            out[v] = (FULL || v < nvalid) ? from_storage(load_raw<NT>(p + int64_t(v) * is * esz, esz), dt) : 0;
     }
 }
 
@@ -55,7 +58,9 @@ __device__ __forceinline__ bool leaf_nt(const RedPlan& p, int l) { return lm_nt(
 __device__ __forceinline__ bool out_nt(const EwPlan& p) { return lm_nt(p.out_mode); }
 __device__ __forceinline__ bool out_nt(const CompactPlan&) { return false; }
 
-template <class S, int I, int V, int NR, int RANK, class P>
+// FULL as for load_at; NTM: non-temporal leaf loads never (0), always for
+// leaves that are not broadcast (1), or per the plan's leaf flags (2)
+template <class S, int I, int V, int NR, int RANK, bool FULL = false, int NTM = 2, class P>
 __device__ __forceinline__ void run_spec(const P& plan, const uint32_t (&idx)[4], uint32_t j0, int nvalid,
                                          Reg (&R)[NR][V]) {
     if constexpr (I < S::nops) {
@@ -65,10 +70,11 @@ __device__ __forceinline__ void run_spec(const P& plan, const uint32_t (&idx)[4]
             constexpr int mode = lm_base(S::leaf_mode[m.aux]);
             const int32_t off = coff<RANK, mode != LM_BCAST>(plan.cst[m.aux], idx, j0);
             const char* p = leaf_ptr(plan, m.aux) + int64_t(off) * dtype_size(dt);
-            if (mode != LM_BCAST && leaf_nt(plan, m.aux))
-                load_at<V, true>(p, mode, dt, plan.cst[m.aux][RANK - 1], nvalid, R[m.dst]);
+            const bool nt = NTM == 2 ? leaf_nt(plan, m.aux) : NTM == 1;
+            if (mode != LM_BCAST && nt)
+                load_at<V, true, FULL>(p, mode, dt, plan.cst[m.aux][RANK - 1], nvalid, R[m.dst]);
             else
-                load_at<V>(p, mode, dt, plan.cst[m.aux][RANK - 1], nvalid, R[m.dst]);
+                load_at<V, false, FULL>(p, mode, dt, plan.cst[m.aux][RANK - 1], nvalid, R[m.dst]);
         } else if constexpr (m.op == MO_GATHER) {
             gather_vals<V>(leaf_ptr(plan, m.aux), S::leaf_dtype[m.aux], plan.leaves[m.aux].stride[0],
                            plan.leaves[m.aux].stride[1], nvalid, R[m.a], R[m.dst]);
@@ -79,7 +85,7 @@ __device__ __forceinline__ void run_spec(const P& plan, const uint32_t (&idx)[4]
 #pragma unroll
             for (int v = 0; v < V; ++v) R[m.dst][v] = tmp[v];
         }
-        run_spec<S, I + 1, V, NR, RANK>(plan, idx, j0, nvalid, R);
+        run_spec<S, I + 1, V, NR, RANK, FULL, NTM>(plan, idx, j0, nvalid, R);
     }
 }
 
