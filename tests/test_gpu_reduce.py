@@ -201,6 +201,34 @@ def test_int32_sum_into_int64_exact(shape):
         assert np.array_equal(got, x.astype(np.int64).sum(axis=ax[0])), k
 
 
+@pytest.mark.parametrize("rows,cols", [(8191, 4096), (1001, 4100), (129, 2050), (7, 4097), (3000, 1023),
+                                       (64, 100003), (2, 1 << 20)])
+def test_fast_kernel_tile_and_split_edges(rows, cols):
+    """Compiled fast kernels around their edges: column tiles of 128 or 64
+    lanes, a tail chunk (cols not a multiple of the vector width), splits whose
+    last row or chunk range is short (rows past the end are clamped and
+    dropped on the whole-tile path).  Integer sums into int64 and maxima are
+    exact; float64 sums within 1e-14 of the long-double oracle."""
+    rng = np.random.default_rng(rows * 7 + cols)
+    xi = rng.integers(-1000, 1001, (rows, cols), dtype=np.int32)
+    XI = Dual.of(xi)
+    xf = rng.normal(0.5, 1, (rows, cols))
+    XF = Dual.of(xf)
+    xs = xf.astype(np.float32)
+    XS = Dual.of(xs)
+    for ax in ([0], [1], None):
+        npax = None if ax is None else ax[0]
+        got, k = run(xt.sum(XI.dev, ax, dtype="int64"))
+        assert np.array_equal(got, xi.astype(np.int64).sum(axis=npax)), (ax, k)
+        got, k = run(xt.amax(XI.dev, ax))
+        assert np.array_equal(got, xi.max(axis=npax)), (ax, k)
+        got, k = run(xt.amax(XS.dev, ax))
+        assert np.array_equal(got, xs.max(axis=npax)), (ax, k)
+        got, k = run(xt.sum(XF.dev, ax))
+        ref = oracle_for(XF).reduce(xt.sum(XF.dev, ax), high_precision=True)
+        assert rel_err(got, ref) <= 1e-14, (ax, k)
+
+
 def test_int32_sum_wraps_like_cpp():
     """sum(int32) has value type int: the total wraps modulo 2^32 exactly as xtensor's does."""
     x = np.full((4, 1 << 16), 1 << 20, np.int32)
