@@ -349,26 +349,37 @@ def serving(np, torch, xt, rates=(1000, 10000, 50000), n_req=300, seed=4):
         return xt.DeviceArray(t.view(torch.uint8), str(t.dtype).replace("torch.", ""), shape)
 
     result = {"requests_per_rate": n_req, "budget_GB": 64, "mean_request_MB": round(float(np.mean(nbytes)) / 1e6, 2)}
+
+    def stream(mode, gaps):
+        outs = [torch.empty(int(np.prod(o)), dtype=getattr(torch, dt), device="cuda") for _, dt, _, o in reqs]
+        cfg = dict(budget_bytes=64 << 30, queue_capacity=1 << 20)
+        cfg.update(dict(streams=4, max_batch=64) if mode == "batched" else dict(streams=1, max_batch=1, plain=True))
+        torch.cuda.synchronize()
+        with Server(**cfg) as srv:
+            t0 = now_ns() + 2_000_000
+            arr = t0 + np.cumsum(gaps * 1e9).astype(np.int64)
+            for k, (tpl, dt, shape, oshape) in enumerate(reqs):
+                while now_ns() < arr[k]:
+                    pass
+                x = view(pools[(tpl, dt)][:int(np.prod(shape))], shape)
+                o = view(outs[k], oshape)
+                ps = [view(p, (16,)) for p in params[dt]] if tpl == "affine_clip" else []
+                srv.submit(tpl, x, o, ps, tag=k, arrival_ns=int(arr[k]))
+            srv.drain(600_000)
+            comps = srv.poll(n_req + 16)
+            st = srv.stats()
+        del outs
+        return arr, comps, st
+
+    # one untimed pass of the whole request list per mode first: the first
+    # request of each shape class pays one-time set-up (run-time compiled
+    # kernels, workspace growth) that a running server has behind it
+    for mode in ("batched", "plain"):
+        stream(mode, np.zeros(n_req))
     for rate in rates:
         gaps = np.random.default_rng(seed + rate).exponential(1.0 / rate, n_req)
         for mode in ("batched", "plain"):
-            outs = [torch.empty(int(np.prod(o)), dtype=getattr(torch, dt), device="cuda") for _, dt, _, o in reqs]
-            cfg = dict(budget_bytes=64 << 30, queue_capacity=1 << 20)
-            cfg.update(dict(streams=4, max_batch=64) if mode == "batched" else dict(streams=1, max_batch=1, plain=True))
-            torch.cuda.synchronize()
-            with Server(**cfg) as srv:
-                t0 = now_ns() + 2_000_000
-                arr = t0 + np.cumsum(gaps * 1e9).astype(np.int64)
-                for k, (tpl, dt, shape, oshape) in enumerate(reqs):
-                    while now_ns() < arr[k]:
-                        pass
-                    x = view(pools[(tpl, dt)][:int(np.prod(shape))], shape)
-                    o = view(outs[k], oshape)
-                    ps = [view(p, (16,)) for p in params[dt]] if tpl == "affine_clip" else []
-                    srv.submit(tpl, x, o, ps, tag=k, arrival_ns=int(arr[k]))
-                srv.drain(600_000)
-                comps = srv.poll(n_req + 16)
-                st = srv.stats()
+            arr, comps, st = stream(mode, gaps)
             lat = np.array([c["done_ns"] - c["arrival_ns"] for c in comps]) / 1e6
             span = (max(c["done_ns"] for c in comps) - int(arr[0])) / 1e9
             result[f"{mode}_{rate}"] = {
@@ -378,7 +389,6 @@ def serving(np, torch, xt, rates=(1000, 10000, 50000), n_req=300, seed=4):
                 "ok": int(sum(c["status"] == 0 for c in comps)),
             }
             print(f"served stream {mode} at {rate} req/s: {result[f'{mode}_{rate}']}", file=sys.stderr, flush=True)
-            del outs
     return result
 
 
