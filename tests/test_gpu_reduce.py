@@ -229,6 +229,30 @@ def test_fast_kernel_tile_and_split_edges(rows, cols):
         assert rel_err(got, ref) <= 1e-14, (ax, k)
 
 
+def test_column_split_merge_repeats_and_streams():
+    """Split column sums (partials in the per-stream workspace, merged by a
+    second kernel): repeated launches, two shapes interleaved, and launches on
+    two streams at once all give the exact int64 sums."""
+    import torch
+    rng = np.random.default_rng(5)
+    xs = [rng.integers(-1000, 1001, s, dtype=np.int32) for s in ((8192, 4096), (3001, 1536))]
+    X = [Dual.of(x) for x in xs]
+    refs = [x.astype(np.int64).sum(axis=0) for x in xs]
+    outs = [xt.empty((x.shape[1],), "int64") for x in xs]
+    plans = [xt.plan(o, xt.sum(d.dev, [0], dtype="int64")) for o, d in zip(outs, X)]
+    for rep in range(6):
+        plans[rep % 2].run()
+        sync()
+        assert np.array_equal(outs[rep % 2].to_numpy(), refs[rep % 2]), rep
+    streams = [torch.cuda.Stream() for _ in range(2)]
+    for rep in range(3):
+        for k, st in enumerate(streams):
+            plans[k].run(stream=st.cuda_stream)
+        torch.cuda.synchronize()
+        for k in range(2):
+            assert np.array_equal(outs[k].to_numpy(), refs[k]), (rep, k)
+
+
 def test_int32_sum_wraps_like_cpp():
     """sum(int32) has value type int: the total wraps modulo 2^32 exactly as xtensor's does."""
     x = np.full((4, 1 << 16), 1 << 20, np.int32)
