@@ -281,6 +281,28 @@ def reductions(np, torch, xt, steps=12, copies=3):
     # as it would in the pipeline c -> sum(c) -> amax(c)
     run("sum_axis1_of_c_f64", lambda d: xt.sum(d, [1]), [c], c_ref.sum(1), 2048 * 1024 * 8)
     run("amax_axis0_of_c_f64", lambda d: xt.amax(d, [0]), [c], c.to_numpy().max(0), 2048 * 1024 * 8, exact=True)
+    # the whole pipeline c, s = sum(c, {1}), m = amax(c, {0}): three plan
+    # launches per step, or one replay of the same plans as a HIP graph
+    s1, m1 = xt.empty((2048,), "float64"), xt.empty((1024,), "float64")
+    seq = [pc, xt.plan(s1, xt.sum(c, [1])), xt.plan(m1, xt.amax(c, [0]))]
+
+    def per_step(fn, n=50):
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    eager_ms = per_step(lambda: [p.run() for p in seq])
+    graph = xt.PlanGraph(seq)
+    graph_ms = per_step(graph.replay)
+    out["pipeline_c_sum_amax_f64"] = {"launches": len(seq), "eager_ms_per_step": round(eager_ms, 4),
+                                      "graph_replay_ms_per_step": round(graph_ms, 4)}
     return out
 
 

@@ -22,6 +22,7 @@ from .core import (  # noqa: F401
 
 from . import blockwise  # noqa: F401,E402
 from .batch import Batch  # noqa: F401,E402
+from .graph import PlanGraph  # noqa: F401,E402
 from .core import execute_many  # noqa: F401,E402
 from .serve import Server  # noqa: F401,E402
 
