@@ -132,3 +132,46 @@ def test_scan_of_expression_f64():
     got = xt.cumsum(e, 1).to_numpy()
     ref = oracle_for(A_).scan(e, 1, high_precision=True)
     assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1)) <= 1e-14
+
+
+@pytest.mark.parametrize("dtype", ["int8", "uint8", "int16", "uint16", "uint32", "uint64", "bfloat16"])
+def test_scans_and_reductions_across_dtypes(dtype):
+    """cumsum / cumprod / sum / prod / amax / mean over each axis of a small
+    array of each less common type, against the oracle: the result type
+    follows xtensor's promotion (the oracle decides it), integers are exact
+    (wrapping as C++ does), bf16 data is compared in float32 accumulation
+    against the oracle's long-double result within 2e-7 relative."""
+    rng = np.random.default_rng(21)
+    shape = (37, 70)
+    if dtype == "bfloat16":
+        X = Dual.of(rng.normal(0.5, 1, shape).astype(np.float32), "bfloat16")
+    elif dtype.startswith("u"):
+        X = Dual.of(rng.integers(0, 9, shape).astype(dtype))
+    else:
+        X = Dual.of(rng.integers(-9, 9, shape).astype(dtype))
+    o = oracle_for(X)
+    for ax in (0, 1, None):
+        for kind in ("cumsum", "cumprod"):
+            got = getattr(xt, kind)(X.dev, ax).to_numpy()
+            if dtype == "bfloat16":
+                if kind == "cumprod":
+                    continue
+                ref = o.scan(X.dev, ax, kind, high_precision=True)
+                assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)) <= 2e-7, (kind, ax)
+            else:
+                ref = o.scan(X.dev, ax, kind)
+                assert got.dtype == ref.dtype and np.array_equal(got, ref), (kind, ax)
+        axes = None if ax is None else [ax]
+        for red in (xt.sum, xt.prod, xt.amax, xt.mean):
+            r = red(X.dev, axes)
+            got = xt.eval(r).to_numpy()
+            if dtype == "bfloat16" and red in (xt.sum, xt.mean):
+                ref = o.reduce(r, high_precision=True)
+                assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)) <= 2e-7, (red.__name__, ax)
+            elif dtype == "bfloat16" and red is xt.prod:
+                continue  # products of 37-70 bf16 values leave float range: not a parity case
+            else:
+                ref = o.reduce(r)
+                assert got.dtype == ref.dtype, (red.__name__, ax, got.dtype, ref.dtype)
+                assert np.array_equal(got, ref) or (got.dtype.kind == "f" and np.allclose(got, ref, rtol=1e-14, atol=0)), \
+                    (red.__name__, ax)
