@@ -163,6 +163,10 @@ hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** use
             const uint64_t tiles = (q.nchunks_f + tw - 1) / tw;
             const uint64_t blocks = tiles * uint64_t(q.splits);
             if (blocks >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
+            // partials [split][output]: a workgroup's partials are one
+            // contiguous run, not one 16-byte write per cache line (1.5-3 %
+            // faster on [8192, 4096] float32 / int32 column sums, MI355X)
+            q.split_major = 1;
             err = go(ks.cols_fast, uint32_t(blocks), 512, q);
         }
         if (err != hipSuccess || q.splits == 1) return err;

@@ -71,7 +71,7 @@ struct RedPlan {
     uint64_t identity;  // start value of every accumulator, in acc representation
     uint64_t initial;   // xt::initial, in acc representation
     int32_t mdt;        // FZ_MEAN / FZ_MEAN_SQRT: type of value / divisor
-    int32_t pad0_;
+    int32_t split_major;  // partials stored [split][output] (cols fast kernel), else [output][split]
     uint64_t div_bits;  // the divisor, in register form of mdt
     char* out_ptr;
     int64_t out_stride[kMaxRank];  // over the kept dims in normalised order (fast last for RL_COLS)

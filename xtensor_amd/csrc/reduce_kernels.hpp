@@ -284,12 +284,13 @@ __global__ __launch_bounds__(256) void reduce_combine_kernel(const RedPlan p) {
         A a;
         a.init(p.identity);
         if (o < p.n_out) {
-            const Store* part = reinterpret_cast<const Store*>(p.partials) + size_t(o) * p.splits;
+            const size_t ps = p.split_major ? size_t(p.n_out) : 1;  // partial stride between splits
+            const Store* part = reinterpret_cast<const Store*>(p.partials) + (p.split_major ? size_t(o) : size_t(o) * p.splits);
             uint32_t s = t;
             for (; s + 7 * T < uint32_t(p.splits); s += 8 * T) {
                 Store st[8];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) st[k] = part[s + uint32_t(k) * T];
+                for (int k = 0; k < 8; ++k) st[k] = part[size_t(s + uint32_t(k) * T) * ps];
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     A b;
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(256) void reduce_combine_kernel(const RedPlan p) {
             }
             for (; s < uint32_t(p.splits); s += T) {
                 A b;
-                b.load(part[s]);
+                b.load(part[size_t(s) * ps]);
                 a.merge(b);
             }
         }
@@ -318,14 +319,15 @@ __global__ __launch_bounds__(256) void reduce_combine_wide_kernel(const RedPlan 
     constexpr int B = 16;
     __shared__ Store lds[4];
     for (uint32_t o = blockIdx.x; o < p.n_out; o += gridDim.x) {
-        const Store* part = reinterpret_cast<const Store*>(p.partials) + size_t(o) * p.splits;
+        const size_t ps = p.split_major ? size_t(p.n_out) : 1;  // partial stride between splits
+        const Store* part = reinterpret_cast<const Store*>(p.partials) + (p.split_major ? size_t(o) : size_t(o) * p.splits);
         A a;
         a.init(p.identity);
         int s = int(threadIdx.x);
         for (; s + (B - 1) * 256 < p.splits; s += B * 256) {
             Store st[B];
 #pragma unroll
-            for (int k = 0; k < B; ++k) st[k] = part[s + k * 256];
+            for (int k = 0; k < B; ++k) st[k] = part[size_t(s + k * 256) * ps];
 #pragma unroll
             for (int k = 0; k < B; ++k) {
                 A b;
@@ -335,7 +337,7 @@ __global__ __launch_bounds__(256) void reduce_combine_wide_kernel(const RedPlan 
         }
         for (; s < p.splits; s += 256) {
             A b;
-            b.load(part[s]);
+            b.load(part[size_t(s) * ps]);
             a.merge(b);
         }
         a = group_combine(a, 256, lds);
@@ -522,6 +524,7 @@ __global__ __launch_bounds__(512) void reduce_cols_fast_kernel(const RedPlan p) 
                 }
                 const uint32_t o = j0 + uint32_t(v);
                 if (p.splits == 1) finish(p, a, o);
+                else if (p.split_major) reinterpret_cast<Store*>(p.partials)[size_t(sp) * p.n_out + o] = a.store();
                 else reinterpret_cast<Store*>(p.partials)[size_t(o) * p.splits + sp] = a.store();
             }
         }
