@@ -7,6 +7,8 @@
 // reducers/xreducer.hpp:1426-1440 and reducers/xaccumulator.hpp:231-234).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -250,15 +252,88 @@ xg_status xg_expr_info(const xg_expr* e, int32_t* dtype, int32_t* ndim, int64_t*
     });
 }
 
+namespace {
+// An assignment whose offsets need 64 bits, cut along the output's first
+// dimension into pieces whose offsets fit 32 bits, so that every piece runs
+// a compiled (or run-time compiled) kernel instead of the generic
+// interpreter.  Each piece slices the output and every leaf that spans that
+// dimension; leaves broadcast along it (missing or of extent 1) stay whole.
+// Returns false, doing nothing, when the expression reads through a gather
+// (its source is addressed by value, not by position) or one index of the
+// first dimension alone already needs 64-bit offsets.
+bool assign_in_pieces(const xg_expr* e, const xg_tensor* out, hipStream_t st, const char** used) {
+    if (out->ndim < 1 || out->shape[0] < 2) return false;
+    for (int i = 0; i < e->n_nodes; ++i)
+        if (e->nodes[i].op == XG_OP_GATHER) return false;
+    auto spans_dim0 = [&](const xg_tensor& t) {
+        const int d = t.ndim - out->ndim;  // the leaf's axis aligned with the output's first
+        return d >= 0 && t.shape[d] != 1 ? d : -1;
+    };
+    // element offsets of one index along the first dimension, and its step
+    int64_t rest = 0, step0 = 0;
+    auto account = [&](const xg_tensor& t, int d0) {
+        int64_t r = 0;
+        for (int k = d0 + 1; k < t.ndim; ++k) r += (std::max<int64_t>(t.shape[k], 1) - 1) * std::llabs(t.strides[k]);
+        rest = std::max(rest, r);
+        step0 = std::max<int64_t>(step0, std::llabs(t.strides[d0]));
+    };
+    account(*out, 0);
+    for (int l = 0; l < e->n_leaves; ++l) {
+        const int d = spans_dim0(e->leaves[l]);
+        if (d >= 0) account(e->leaves[l], d);
+    }
+    constexpr int64_t kLim = (int64_t(1) << 31) - 1;
+    if (rest >= kLim) return false;
+    int64_t rows = step0 > 0 ? (kLim - rest) / step0 + 1 : out->shape[0];
+    rows = std::min<int64_t>(rows, int64_t(1) << 30);  // and the inner extent stays below 2^31
+    if (rows >= out->shape[0]) return false;
+    std::vector<xg_tensor> leaves(e->leaves, e->leaves + e->n_leaves);
+    xg_expr pe = *e;
+    pe.leaves = leaves.data();
+    for (int64_t s0 = 0; s0 < out->shape[0]; s0 += rows) {
+        const int64_t len = std::min(rows, out->shape[0] - s0);
+        xg_tensor o = *out;
+        o.offset += s0 * out->strides[0];
+        o.shape[0] = len;
+        for (int l = 0; l < e->n_leaves; ++l) {
+            leaves[size_t(l)] = e->leaves[l];
+            const int d = spans_dim0(e->leaves[l]);
+            if (d >= 0) {
+                leaves[size_t(l)].offset += s0 * e->leaves[l].strides[d];
+                leaves[size_t(l)].shape[d] = len;
+            }
+        }
+        int V = 4;
+        EwPlan p = plan_assign_best(&pe, &o, &V);
+        if (p.units) check_hip(launch_ew(p, V, device_cus(), st, used), "elementwise launch");
+    }
+    return true;
+}
+}  // namespace
+
 xg_status xg_assign(const xg_expr* e, const xg_tensor* out, xg_stream_t stream) {
     return guard([&] {
         int V = 4;
-        EwPlan p = plan_assign_best(e, out, &V);
+        EwPlan p{};
+        bool whole = true;  // false: the iteration space does not fit one plan's 32-bit unit indexing
+        Error refused{};
+        try {
+            p = plan_assign_best(e, out, &V);
+        } catch (const Error& err) {
+            if (err.code != XG_UNSUPPORTED) throw;
+            whole = false;
+            refused = err;
+        }
+        const char* used = nullptr;
+        if ((!whole || p.index64) && assign_in_pieces(e, out, reinterpret_cast<hipStream_t>(stream), &used)) {
+            note_kernel(used ? used : "empty");
+            return;
+        }
+        if (!whole) throw refused;
         if (p.units == 0) {
             note_kernel("empty");
             return;
         }
-        const char* used = nullptr;
         check_hip(launch_ew(p, V, device_cus(), reinterpret_cast<hipStream_t>(stream), &used), "elementwise launch");
         note_kernel(used);
     });
