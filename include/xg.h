@@ -274,7 +274,9 @@ xg_status xg_expr_info(const xg_expr* e, int32_t* dtype, int32_t* ndim, int64_t*
  * The broadcast shape of e must be broadcastable to out's shape
  * (xexpression_assigner::assert_compatible_shape).  Values are converted to
  * out's dtype with static_cast (conditional_cast).  Outputs may alias inputs
- * only element-for-element (out[i] read and written by the same element). */
+ * only element-for-element (out[i] read and written by the same element).
+ * Arrays past 2^31 elements (or offsets) are accepted: the work is enqueued
+ * as several launches along out's first dimension. */
 xg_status xg_assign(const xg_expr* e, const xg_tensor* out, xg_stream_t stream);
 
 /* Result dtype and shape of a reduction of e. */
