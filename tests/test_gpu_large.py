@@ -24,8 +24,16 @@ def test_flat_int8_past_2_31():
     xt.noalias(Y).assign(X + 3)
     torch.cuda.synchronize()
     assert xt.last_kernel() != "dynamic"
-    assert torch.equal(y, (x.to(torch.int32) + 3).to(torch.int8))
-    del x, y
+    ref = (x.to(torch.int32) + 3).to(torch.int8)
+    assert torch.equal(y, ref)
+    # the same as a prepared plan, run alone and through the fused entry point
+    pl = xt.plan(Y, X + 3)
+    for run in (lambda: pl.run(), lambda: xt.execute_many([pl])):
+        y.zero_()
+        run()
+        torch.cuda.synchronize()
+        assert torch.equal(y, ref)
+    del x, y, ref, pl
     torch.cuda.empty_cache()
 
 

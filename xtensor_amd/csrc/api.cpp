@@ -12,6 +12,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include <map>
@@ -258,10 +259,11 @@ namespace {
 // a compiled (or run-time compiled) kernel instead of the generic
 // interpreter.  Each piece slices the output and every leaf that spans that
 // dimension; leaves broadcast along it (missing or of extent 1) stay whole.
-// Returns false, doing nothing, when the expression reads through a gather
-// (its source is addressed by value, not by position) or one index of the
-// first dimension alone already needs 64-bit offsets.
-bool assign_in_pieces(const xg_expr* e, const xg_tensor* out, hipStream_t st, const char** used) {
+// Returns false, planning nothing, when the expression reads through a
+// gather (its source is addressed by value, not by position) or one index
+// of the first dimension alone already needs 64-bit offsets.
+using Piece = std::pair<EwPlan, int>;  // a plan and its vector width
+bool plan_in_pieces(const xg_expr* e, const xg_tensor* out, std::vector<Piece>* pieces) {
     if (out->ndim < 1 || out->shape[0] < 2) return false;
     for (int i = 0; i < e->n_nodes; ++i)
         if (e->nodes[i].op == XG_OP_GATHER) return false;
@@ -305,7 +307,7 @@ bool assign_in_pieces(const xg_expr* e, const xg_tensor* out, hipStream_t st, co
         }
         int V = 4;
         EwPlan p = plan_assign_best(&pe, &o, &V);
-        if (p.units) check_hip(launch_ew(p, V, device_cus(), st, used), "elementwise launch");
+        if (p.units) pieces->emplace_back(p, V);
     }
     return true;
 }
@@ -325,7 +327,11 @@ xg_status xg_assign(const xg_expr* e, const xg_tensor* out, xg_stream_t stream) 
             refused = err;
         }
         const char* used = nullptr;
-        if ((!whole || p.index64) && assign_in_pieces(e, out, reinterpret_cast<hipStream_t>(stream), &used)) {
+        std::vector<Piece> pieces;
+        if ((!whole || p.index64) && plan_in_pieces(e, out, &pieces)) {
+            for (const Piece& pc : pieces)
+                check_hip(launch_ew(pc.first, pc.second, device_cus(), reinterpret_cast<hipStream_t>(stream), &used),
+                          "elementwise launch");
             note_kernel(used ? used : "empty");
             return;
         }
@@ -452,6 +458,7 @@ struct xg_plan_s {
     int kind = 0;  // 0 assign, 1 reduce, 2 scan
     int V = 4;
     xg::EwPlan ew;
+    std::vector<std::pair<xg::EwPlan, int>> pieces;  // assign past 32-bit offsets: one plan per piece (ew unused)
     xg::RedPlan red;
     xg::ScanPlan scan;
     size_t ws = 0;
@@ -464,7 +471,17 @@ xg_status xg_plan_assign(const xg_expr* e, const xg_tensor* out, xg_plan_t* plan
         auto* p = new xg_plan_s();
         p->kind = 0;
         try {
-            p->ew = plan_assign_best(e, out, &p->V);
+            bool whole = true;
+            Error refused{};
+            try {
+                p->ew = plan_assign_best(e, out, &p->V);
+            } catch (const Error& err) {
+                if (err.code != XG_UNSUPPORTED) throw;
+                whole = false;
+                refused = err;
+            }
+            if ((!whole || p->ew.index64) && plan_in_pieces(e, out, &p->pieces)) p->ew.units = 0;
+            else if (!whole) throw refused;
         } catch (...) {
             delete p;
             throw;
@@ -517,6 +534,7 @@ xg_status xg_plan_execute(xg_plan_t p, void* ws, xg_stream_t stream) {
         const char* used = nullptr;
         if (p->kind == 0) {
             if (p->ew.units) check_hip(launch_ew(p->ew, p->V, device_cus(), st, &used), "elementwise launch");
+            for (const auto& pc : p->pieces) check_hip(launch_ew(pc.first, pc.second, device_cus(), st, &used), "elementwise launch");
         } else if (p->kind == 1) {
             if (p->red.n_out) {
                 p->red.partials = p->ws ? (ws ? ws : workspace(p->ws, st)) : nullptr;
@@ -540,6 +558,10 @@ xg_status xg_plan_execute_many(const xg_plan_t* plans, int64_t n, xg_stream_t st
             if (plans[i]->kind == 0) {
                 ew.push_back(&plans[i]->ew);
                 V.push_back(plans[i]->V);
+                for (const auto& pc : plans[i]->pieces) {
+                    ew.push_back(&pc.first);
+                    V.push_back(pc.second);
+                }
             }
         }
         launch_ew_many(ew.data(), V.data(), ew.size(), st);
