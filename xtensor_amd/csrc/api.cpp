@@ -627,4 +627,32 @@ xg_status xg_debug_assign_kernel(const xg_expr* e, const xg_tensor* out, char* n
     });
 }
 
+// Testing aid (host only, no device access): how an assignment past 32-bit
+// offsets is cut.  *n_pieces = the number of pieces (0: not cut), *all_compact
+// = 1 if every piece fits 32-bit offsets, *max_units = the largest piece's
+// work units.  Planned with compiled and run-time compiled programs off.
+xg_status xg_debug_assign_pieces(const xg_expr* e, const xg_tensor* out, int64_t* n_pieces, int32_t* all_compact,
+                                 int64_t* max_units) {
+    return guard([&] {
+        if (!n_pieces || !all_compact || !max_units) raise(XG_INVALID_ARGUMENT, "null output pointer");
+        ew_set_force_dynamic(1);
+        std::vector<Piece> pieces;
+        bool ok = false;
+        try {
+            ok = plan_in_pieces(e, out, &pieces);
+        } catch (...) {
+            ew_set_force_dynamic(0);
+            throw;
+        }
+        ew_set_force_dynamic(0);
+        *n_pieces = ok ? int64_t(pieces.size()) : 0;
+        *all_compact = 1;
+        *max_units = 0;
+        for (const Piece& pc : pieces) {
+            if (!pc.first.compact) *all_compact = 0;
+            *max_units = std::max<int64_t>(*max_units, int64_t(pc.first.units));
+        }
+    });
+}
+
 }  // extern "C"
