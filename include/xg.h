@@ -285,7 +285,9 @@ xg_status xg_reduce_info(const xg_expr* e, const xg_reduce_desc* r, int32_t* dty
 
 /* out = reduce(e, axes)   (xt::sum/prod/amax/amin/mean/norm_* assigned into out).
  * out must have exactly the reduction's shape; its dtype may differ from the
- * result dtype (then values are static_cast on store). */
+ * result dtype (then values are static_cast on store).  sum / prod / amax /
+ * amin accept a reduced dimension past 2^31 elements: it is reduced in
+ * pieces through a temporary of the result's size per piece. */
 xg_status xg_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, xg_stream_t stream);
 
 /* Result dtype and shape of a scan of e. */

@@ -52,3 +52,35 @@ def test_rows_int16_with_broadcast_row_past_2_31():
     assert torch.equal(got, ref) or torch.equal(y, ref.to(torch.int16))
     del x, b, y, ref, got
     torch.cuda.empty_cache()
+
+
+def test_reductions_over_a_dimension_past_2_31():
+    """sum / amax over a reduced dimension of more than 2^31 elements: pieces
+    of 2^30 reduce into a temporary, and a second reduction merges them.
+    Integer sums into int64 and maxima are exact against torch; a float32
+    sum of uniform(0, 1) data is within 2e-7 of torch's float64 sum."""
+    import torch
+    n = (1 << 31) + (1 << 21) + 3
+    x = torch.randint(-100, 100, (n,), dtype=torch.int8, device="cuda")
+    X = wrap(x, (n,))
+    got = xt.eval(xt.sum(X, None, dtype="int64")).to_numpy()
+    assert int(got) == int(x.sum(dtype=torch.int64).item())
+    got = xt.eval(xt.amax(X)).to_numpy()
+    assert int(got) == int(x.max().item())
+    # into an output narrower than the int64 result: the pieces stay int64, the store casts
+    o32 = xt.empty((), "int32")
+    xt.noalias(o32).assign(xt.sum(X, None, dtype="int64"))
+    assert int(o32.to_numpy()) == int(x.sum(dtype=torch.int64).item())
+    del x, X
+    c = (1 << 31) + 5
+    x2 = torch.randint(-100, 100, (2, c), dtype=torch.int8, device="cuda")
+    got = xt.eval(xt.sum(wrap(x2, (2, c)), [1], dtype="int64")).to_numpy()
+    assert got.tolist() == x2.sum(1, dtype=torch.int64).tolist()
+    del x2
+    m = (1 << 31) + 7
+    f = torch.rand(m, dtype=torch.float32, device="cuda")
+    got = float(xt.eval(xt.sum(wrap(f, (m,)), None)).to_numpy())
+    ref = float(f.double().sum().item())
+    assert abs(got - ref) / ref <= 2e-7, (got, ref)
+    del f
+    torch.cuda.empty_cache()

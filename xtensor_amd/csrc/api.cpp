@@ -357,6 +357,115 @@ xg_status xg_reduce_info(const xg_expr* e, const xg_reduce_desc* r, int32_t* dty
     });
 }
 
+namespace {
+// sum / prod / amax / amin over a reduced dimension of 2^31 elements or more
+// (beyond 32-bit indexing): that dimension is cut into pieces of 2^30, each
+// piece reduces into its own slice of a temporary of out's shape (in the
+// reduction's result type), and a second reduction of the same kind over
+// the pieces gives out (with xt::initial applied there, once).  Returns false, doing nothing, for other
+// reductions, gathers, or a second dimension that is itself too long.
+bool reduce_in_pieces(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t st) {
+    if (!r || !out || (r->op != XG_RED_SUM && r->op != XG_RED_PROD && r->op != XG_RED_AMAX && r->op != XG_RED_AMIN))
+        return false;
+    for (int i = 0; i < e->n_nodes; ++i)
+        if (e->nodes[i].op == XG_OP_GATHER) return false;
+    const ExprInfo info = analyse(e);
+    const int R = int(info.shape.size());
+    constexpr int64_t kBig = int64_t(1) << 31;
+    std::vector<bool> red(size_t(R), r->n_axes < 0);
+    for (int i = 0; i < r->n_axes; ++i) {
+        const int a = r->axes[i] < 0 ? r->axes[i] + R : r->axes[i];
+        if (a < 0 || a >= R) return false;
+        red[size_t(a)] = true;
+    }
+    int d = -1;
+    for (int k = 0; k < R; ++k) {
+        if (info.shape[size_t(k)] < kBig) continue;
+        if (!red[size_t(k)] || d >= 0) return false;
+        d = k;
+    }
+    if (d < 0) return false;
+    int64_t out_elems = 1;
+    for (int k = 0; k < out->ndim; ++k) out_elems *= out->shape[k];
+    if (out_elems <= 0) return false;
+    const int64_t n = info.shape[size_t(d)], piece = int64_t(1) << 30, k_pieces = (n + piece - 1) / piece;
+    // the pieces keep the reduction's own result type (out's may be narrower: cast on the final store)
+    int rdt = 0;
+    Shape rshape;
+    reduce_result(e, r, &rdt, &rshape);
+    const int esz = dtype_size(rdt);
+    void* tmp = nullptr;
+    if (hipMalloc(&tmp, size_t(k_pieces * out_elems * esz)) != hipSuccess)
+        raise(XG_OUT_OF_MEMORY, "temporary for a reduction in pieces");
+    try {
+        std::vector<xg_tensor> leaves(e->leaves, e->leaves + e->n_leaves);
+        xg_expr pe = *e;
+        pe.leaves = leaves.data();
+        xg_reduce_desc pr = *r;
+        pr.has_initial = 0;
+        xg_tensor o = *out;  // one slice of the temporary, contiguous, of out's shape
+        o.data = tmp;
+        o.dtype = rdt;
+        for (int64_t k = o.ndim - 1, stride = 1; k >= 0; --k) {
+            o.strides[k] = stride;
+            stride *= o.shape[k];
+        }
+        const char* used = nullptr;
+        for (int64_t i = 0; i < k_pieces; ++i) {
+            const int64_t s0 = i * piece, len = std::min(piece, n - s0);
+            for (int l = 0; l < e->n_leaves; ++l) {
+                const xg_tensor& t = e->leaves[l];
+                leaves[size_t(l)] = t;
+                const int dl = d - (R - t.ndim);  // the leaf's axis aligned with d
+                if (dl >= 0 && t.shape[dl] != 1) {
+                    leaves[size_t(l)].offset += s0 * t.strides[dl];
+                    leaves[size_t(l)].shape[dl] = len;
+                }
+            }
+            o.offset = i * out_elems;
+            int V = 4;
+            RedPlan p = plan_reduce_best(&pe, &pr, &o, &V);
+            p.partials = workspace(reduce_workspace_bytes(p), st);
+            check_hip(launch_reduce(p, V, st, &used), "reduction launch");
+        }
+        // the pieces' results as [pieces, out shape...], reduced over axis 0 into out
+        xg_tensor parts{};
+        parts.data = tmp;
+        parts.dtype = rdt;
+        parts.ndim = out->ndim + 1;
+        parts.shape[0] = k_pieces;
+        parts.strides[0] = out_elems;
+        for (int k = 0; k < out->ndim; ++k) {
+            parts.shape[k + 1] = o.shape[k];
+            parts.strides[k + 1] = o.strides[k];
+        }
+        xg_node leaf{};
+        leaf.op = XG_OP_LEAF;
+        xg_expr me{};
+        me.n_nodes = 1;
+        me.n_leaves = 1;
+        me.nodes = &leaf;
+        me.leaves = &parts;
+        xg_reduce_desc mr = *r;
+        mr.n_axes = 1;
+        mr.axes[0] = 0;
+        mr.keep_dims = 0;
+        int V = 4;
+        RedPlan p = plan_reduce_best(&me, &mr, out, &V);
+        p.partials = workspace(reduce_workspace_bytes(p), st);
+        check_hip(launch_reduce(p, V, st, &used), "reduction launch");
+        note_kernel(used);
+        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(tmp);
+        throw;
+    }
+    check_hip(hipFree(tmp), "hipFree");
+    return true;
+}
+}  // namespace
+
 xg_status xg_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, xg_stream_t stream) {
     return guard([&] {
         hipStream_t st = reinterpret_cast<hipStream_t>(stream);
@@ -369,7 +478,13 @@ xg_status xg_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
             return;
         }
         int V = 4;
-        RedPlan p = plan_reduce_best(e, r, out, &V);
+        RedPlan p{};
+        try {
+            p = plan_reduce_best(e, r, out, &V);
+        } catch (const Error& err) {
+            if (err.code != XG_UNSUPPORTED || !reduce_in_pieces(e, r, out, st)) throw;
+            return;
+        }
         if (p.n_out == 0) {
             note_kernel("empty");
             return;
