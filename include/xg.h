@@ -222,6 +222,13 @@ typedef enum xg_reduce_op {
                           /* core/xmath.hpp:2195-2230                          */
 } xg_reduce_op;
 
+/* xt::evaluation_strategy::immediate (reducers/xreducer.hpp:35-161, 944-952):
+ * the expression is evaluated into a temporary first (a lone container is
+ * reduced as it is) and the temporary is reduced.  Values are those of the
+ * default lazy form: integers exactly, floating-point sums within the same
+ * error bound (the device's summation order is its own in both forms). */
+#define XG_REDUCE_IMMEDIATE 1
+
 typedef struct xg_reduce_desc {
     int32_t op;           /* xg_reduce_op                                    */
     int32_t acc_dtype;    /* the template argument T, or -1 for the default  */
@@ -230,7 +237,7 @@ typedef struct xg_reduce_desc {
     int32_t keep_dims;    /* xt::keep_dims option                            */
     int32_t has_initial;  /* xt::initial(value) option                       */
     xg_scalar initial;
-    int32_t flags;        /* reserved, 0                                      */
+    int32_t flags;        /* 0, or XG_REDUCE_IMMEDIATE                        */
     int32_t ddof;         /* variance / stddev: delta degrees of freedom; the */
                           /* divisor is (size - ddof) / outputs in size_t     */
                           /* arithmetic, like detail::mean; 0 for other ops    */

@@ -742,6 +742,181 @@ void reduce_loop(Evaluator& ev, const Shape& in, const std::vector<int>& ax, boo
     }
 }
 
+// True when e is one column-major container (strides increasing from the
+// first dimension, extent-1 dimensions ignored) that is not also row-major:
+// xt::eval returns such a container as it is, in its own layout.
+bool column_major_container(const xg_expr* e) {
+    if (e->n_nodes != 1 || e->nodes[0].op != XG_OP_LEAF) return false;
+    const xg_tensor& t = e->leaves[e->nodes[0].index];
+    auto contiguous = [&](bool col) {
+        int64_t run = 1;
+        for (int i = 0; i < t.ndim; ++i) {
+            const int k = col ? i : t.ndim - 1 - i;
+            if (t.shape[k] == 1) continue;
+            if (t.strides[k] != run) return false;
+            run *= t.shape[k];
+        }
+        return true;
+    };
+    return contiguous(true) && !contiguous(false);
+}
+
+// reduce_immediate restated (reducers/xreducer.hpp:289-565), the path of
+// evaluation_strategy::immediate (reduce_impl, :944-952): e is evaluated
+// first (xt::eval, core/xeval.hpp:45-57: a column-major container stays
+// as it is, anything else becomes a row-major temporary) and reduced over
+// that container's storage, with the result laid out like e.
+//   * no axes: every element v becomes f(init, v); xt::initial is not used
+//   * every axis: std::accumulate over the storage from `initial` (or init)
+//   * otherwise the leading reduced axis (last for row-major, first for
+//     column-major) gives the inner stride; adjacent reduced axes beyond it
+//     merge into the outer loop; with inner stride 1 each run of the outer
+//     loop is accumulated and merged into its output, else the outputs of
+//     one inner run are folded element by element down the outer loop; an
+//     output is merged into when the odometer over the remaining axes comes
+//     back to it; xt::initial is merged in last, behind the value
+// Where e holds no element the reference walks storage it does not have;
+// the restatement gives every output the start value instead.
+template <class VAL, class FN, class MG>
+void reduce_immediate_loop(Evaluator& ev, const Shape& in, const std::vector<int>& ax, bool keep, bool col_major,
+                           VAL init, bool has_initial, VAL initial, FN fn, MG mg,
+                           const std::function<void(int64_t, VAL)>& emit) {
+    const int nd = int(in.size());
+    // strides of a container of shape s in e's layout, 0 along extent-1 dims
+    auto layout_strides = [&](const Shape& sh) {
+        std::vector<int64_t> st(sh.size(), 0);
+        int64_t run = 1;
+        for (size_t i = 0; i < sh.size(); ++i) {
+            const size_t k = col_major ? i : sh.size() - 1 - i;
+            st[k] = sh[k] == 1 ? 0 : run;
+            run *= sh[k];
+        }
+        return st;
+    };
+    const std::vector<int64_t> es = layout_strides(in);
+    const Shape rs = reduce_shape(in, ax, keep);
+    const std::vector<int64_t> rst = layout_strides(rs);
+    const int64_t total = numel(in), n_out = numel(rs);
+    std::vector<VAL> res(size_t(std::max<int64_t>(n_out, 1)), init);
+    auto emit_all = [&] {
+        std::vector<int64_t> ridx(rs.size(), 0);
+        for (int64_t o = 0; o < n_out; ++o) {
+            int64_t pos = 0;
+            for (size_t k = 0; k < rs.size(); ++k) pos += ridx[k] * rst[k];
+            emit(o, res[size_t(pos)]);
+            next_index(ridx, rs);
+        }
+    };
+    if (total == 0) {
+        for (auto& v : res) v = has_initial ? mg(init, initial) : init;
+        emit_all();
+        return;
+    }
+    // the evaluated expression's storage
+    std::vector<Val> data(static_cast<size_t>(total));
+    {
+        std::vector<int64_t> idx(size_t(nd), 0);
+        do {
+            int64_t pos = 0;
+            for (int k = 0; k < nd; ++k) pos += idx[size_t(k)] * es[size_t(k)];
+            data[size_t(pos)] = ev.at(idx.data(), nd);
+        } while (next_index(idx, in));
+    }
+    if (ax.empty()) {
+        for (int64_t i = 0; i < total; ++i) res[size_t(i)] = fn(init, data[size_t(i)]);
+        emit_all();
+        return;
+    }
+    if (int(ax.size()) == nd) {
+        VAL acc = has_initial ? initial : init;
+        for (const Val& v : data) acc = fn(acc, v);
+        res[0] = acc;
+        emit_all();
+        return;
+    }
+    const int leading = col_major ? ax.front() : ax.back();
+    int64_t inner_stride = es[size_t(leading)];
+    for (int k = leading; inner_stride == 0 && (col_major ? k < nd - 1 : k > 0);) {
+        k += col_major ? 1 : -1;
+        inner_stride = es[size_t(k)];
+    }
+    if (inner_stride == 0) {  // every reduced extent is 1: e and the result have the same storage
+        for (int64_t i = 0; i < total; ++i) res[size_t(i)] = fn(init, data[size_t(i)]);
+        emit_all();
+        return;
+    }
+    int64_t outer_loop = in[size_t(leading)];
+    int last_ax = leading;
+    if (col_major) {
+        for (size_t i = 1; i < ax.size(); ++i)
+            if (std::abs(ax[i] - last_ax) == 1) { last_ax = ax[i]; outer_loop *= in[size_t(last_ax)]; }
+    } else {
+        for (int i = int(ax.size()) - 2; i >= 0; --i)
+            if (std::abs(ax[size_t(i)] - last_ax) == 1) { last_ax = ax[size_t(i)]; outer_loop *= in[size_t(last_ax)]; }
+    }
+    std::vector<int64_t> iter_shape(in.begin(), in.end()), iter_strides(size_t(nd), 0);
+    for (int k = 0, oi = 0; k < nd; ++k) {
+        if (std::find(ax.begin(), ax.end(), k) != ax.end()) continue;
+        iter_strides[size_t(k)] = rst[size_t(keep ? k : oi)];
+        ++oi;
+    }
+    if (col_major) {
+        iter_shape.erase(iter_shape.begin(), iter_shape.begin() + last_ax + 1);
+        iter_strides.erase(iter_strides.begin(), iter_strides.begin() + last_ax + 1);
+        std::reverse(iter_shape.begin(), iter_shape.end());
+        std::reverse(iter_strides.begin(), iter_strides.end());
+    } else {
+        iter_shape.erase(iter_shape.begin() + last_ax, iter_shape.end());
+        iter_strides.erase(iter_strides.begin() + last_ax, iter_strides.end());
+    }
+    std::vector<int64_t> t_idx(iter_shape.size(), 0);
+    // odometer over the remaining axes: (wrapped around, output offset)
+    auto next = [&]() {
+        size_t i = iter_shape.size();
+        for (; i > 0; --i) {
+            if (t_idx[i - 1] >= iter_shape[i - 1] - 1) {
+                t_idx[i - 1] = 0;
+            } else {
+                ++t_idx[i - 1];
+                break;
+            }
+        }
+        int64_t off = 0;
+        for (size_t k = 0; k < t_idx.size(); ++k) off += t_idx[k] * iter_strides[k];
+        return std::make_pair(i == 0, off);
+    };
+    int64_t b = 0, o = 0, border = 0;
+    bool merge = false, done = false;
+    while (!done) {
+        if (inner_stride == 1) {
+            VAL acc = init;
+            for (int64_t k = 0; k < outer_loop; ++k) acc = fn(acc, data[size_t(b + k)]);
+            res[size_t(o)] = merge ? mg(res[size_t(o)], acc) : acc;
+            b += outer_loop;
+        } else {
+            for (int64_t k = 0; k < inner_stride; ++k)
+                res[size_t(o + k)] = merge ? fn(res[size_t(o + k)], data[size_t(b + k)]) : fn(init, data[size_t(b + k)]);
+            b += inner_stride;
+            for (int64_t i = 1; i < outer_loop; ++i) {
+                for (int64_t k = 0; k < inner_stride; ++k) res[size_t(o + k)] = fn(res[size_t(o + k)], data[size_t(b + k)]);
+                b += inner_stride;
+            }
+        }
+        auto nx = next();
+        done = nx.first;
+        o = nx.second;
+        if (o > border) {
+            merge = false;
+            border = o;
+        } else {
+            merge = true;
+        }
+    }
+    if (has_initial)
+        for (auto& v : res) v = mg(v, initial);
+    emit_all();
+}
+
 // result dtype of a reduction with element value type `edt`
 int reduce_result_dt(int op, int acc, int edt) {
     return visit(edt, [&](auto te) -> int {
@@ -925,7 +1100,7 @@ void do_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, 
         default: return convert(apply2<f_plus>(a, b), vdt);
         }
     };
-    reduce_loop<Val>(ev, in, ax, keep, init, r->has_initial != 0, ini, fn, mg, [&](int64_t o, Val v) {
+    auto finish = [&](int64_t o, Val v) {
         if (op == XG_RED_MEAN) {
             // mean_division: div = value_type((e.size() - ddof) / s.size()) with
             // value_type = T or double, 0 when s is empty
@@ -937,7 +1112,12 @@ void do_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, 
         }
         if (sqrt_after) v = apply1<f_sqrt>(v);
         store(*out, emit_offset(o), v);
-    });
+    };
+    if (r->flags & XG_REDUCE_IMMEDIATE)
+        reduce_immediate_loop<Val>(ev, in, ax, keep, column_major_container(e), init, r->has_initial != 0, ini, fn,
+                                   mg, finish);
+    else
+        reduce_loop<Val>(ev, in, ax, keep, init, r->has_initial != 0, ini, fn, mg, finish);
 }
 
 // xt::variance / xt::stddev restated (core/xmath.hpp:2031-2113): the inner

@@ -17,6 +17,7 @@ import pytest
 
 import xtensor_amd as xt
 from xtensor_amd import _abi as A
+from xtensor_amd import core
 
 from helpers import Dual, oracle_for
 from oracle.pyoracle import Oracle, OracleError
@@ -514,3 +515,346 @@ def test_high_precision_reference_is_exact_on_integers():
     x = np.arange(1, 10001, dtype=np.float64)
     X = host(x)
     assert oracle_for(X).reduce(xt.sum(X.dev), high_precision=True)[()] == 10000 * 10001 / 2
+
+
+# ---------------------------------------------------------------- prod
+def test_prod_kat_no_overflow():
+    """test_xreducer.cpp:507-512: prod<long long>(2 * ones<uint8_t>({34})) == 1 << 34."""
+    c = host(2 * np.ones(34, np.uint8))
+    o = oracle_for(c)
+    r = xt.prod(c.dev, None, dtype="int64")
+    assert xt.dtype_name(o.reduce_info(r)[0]) == "int64"
+    assert int(o.reduce(r)[()]) == 1 << 34
+
+
+def test_prod_closed_forms():
+    """prod over small integers wraps like C++ unsigned arithmetic in the
+    value type (numpy's integer product, exact); float prod is the
+    sequential left fold in the value type."""
+    rng = np.random.default_rng(21)
+    i = rng.integers(-5, 6, (6, 40), dtype=np.int32)
+    I = host(i)
+    o = oracle_for(I)
+    assert np.array_equal(o.reduce(xt.prod(I.dev, [1])), np.prod(i, axis=1, dtype=np.int32))
+    assert np.array_equal(o.reduce(xt.prod(I.dev, [0], dtype="int64")), np.prod(i.astype(np.int64), axis=0))
+    f = rng.uniform(0.5, 1.5, (5, 33)).astype(np.float32)
+    F = host(f)
+    got = oracle_for(F).reduce(xt.prod(F.dev, [1]))
+    for r in range(5):
+        acc = np.float32(1)
+        for v in f[r]:
+            acc = np.float32(acc * v)
+        assert got[r] == acc
+    # initial is merged once, after the fold: prod(x, initial=3) = 3 * prod(x)
+    small = host(np.array([2, 3, 4], np.int64))
+    assert int(oracle_for(small).reduce(xt.prod(small.dev, None, initial=3))[()]) == 72
+
+
+# ---------------------------------------------------------------- every opcode
+# C++ value types of the operands (xtensor containers hold these), the
+# integral promotions and the usual arithmetic conversions ([conv.prom],
+# [expr.arith.conv]) that decide each operator's result type, restated
+# from the C++ standard for the LP64 types of this platform.
+_INT_RANK = {np.int8: 1, np.uint8: 1, np.int16: 2, np.uint16: 2, np.int32: 3, np.uint32: 3,
+             np.int64: 4, np.uint64: 4}
+
+
+def _promote(t):
+    t = np.dtype(t).type
+    if t in (np.bool_, np.int8, np.uint8, np.int16, np.uint16):
+        return np.int32
+    return t
+
+
+def _usual(a, b):
+    a, b = np.dtype(a).type, np.dtype(b).type
+    if np.float64 in (a, b):
+        return np.float64
+    if np.float32 in (a, b):
+        return np.float32
+    a, b = _promote(a), _promote(b)
+    if a == b:
+        return a
+    sa, sb = np.issubdtype(a, np.signedinteger), np.issubdtype(b, np.signedinteger)
+    if sa == sb:
+        return a if _INT_RANK[a] >= _INT_RANK[b] else b
+    u, s = (b, a) if sa else (a, b)
+    return u if _INT_RANK[u] >= _INT_RANK[s] else s
+
+
+def _operands(dt, rng, n=64, nonzero=False, shift=False):
+    t = np.dtype(dt).type
+    if t == np.bool_:
+        v = rng.integers(0, 2, n).astype(np.bool_)
+    elif np.issubdtype(t, np.floating):
+        v = rng.uniform(-20, 20, n).astype(t)
+        if nonzero:
+            v[v == 0] = 1
+    else:
+        lo = 0 if shift else (max(np.iinfo(t).min, -100) if np.issubdtype(t, np.signedinteger) else 0)
+        hi = 16 if shift else min(np.iinfo(t).max, 100)
+        v = rng.integers(lo, hi, n).astype(t)
+        if nonzero:
+            v[v == 0] = 7
+    return v
+
+
+def _cxx_int_div(a, b):
+    q = abs(int(a)) // abs(int(b))
+    return -q if (a < 0) != (b < 0) else q
+
+
+_PAIRS = [(np.int8, np.uint8), (np.int32, np.uint32), (np.int16, np.int64), (np.uint64, np.int64),
+          (np.uint8, np.float32), (np.int32, np.float64), (np.float32, np.float64), (np.bool_, np.int16),
+          (np.uint16, np.int32)]
+
+
+def _check_binary(op, expect_fn, pairs, rng, result=None, **kw):
+    for ta, tb in pairs:
+        a = _operands(ta, rng, **kw.get("a", {}))
+        b = _operands(tb, rng, **kw.get("b", {}))
+        A_, B_ = host(a), host(b)
+        o = oracle_for(A_, B_)
+        e = core.Func(A.OP[op], A_.dev, B_.dev)
+        rt = result(ta, tb) if result else _usual(ta, tb)
+        assert xt.dtype_name(o.info(e)[0]) == np.dtype(rt).name, (op, ta, tb)
+        got = o.assign(e)
+        want = np.array([expect_fn(x, y, rt) for x, y in zip(a.astype(rt) if rt != np.bool_ else a,
+                                                              b.astype(rt) if rt != np.bool_ else b)], dtype=rt)
+        if result is not None and rt == np.bool_:
+            want = np.array([expect_fn(x, y, rt) for x, y in zip(a, b)], dtype=np.bool_)
+        assert np.array_equal(got, want), (op, ta, tb)
+
+
+def _wrap_int(v, rt):
+    return np.array(v).astype(np.int64).astype(rt) if v >= 0 or np.issubdtype(rt, np.signedinteger) else \
+        np.array(int(v) % (1 << (8 * np.dtype(rt).itemsize)), dtype=np.uint64).astype(rt)
+
+
+def test_arithmetic_opcodes_closed_form():
+    rng = np.random.default_rng(31)
+    for op, f in (("PLUS", np.add), ("MINUS", np.subtract), ("MULTIPLIES", np.multiply)):
+        _check_binary(op, lambda x, y, rt, f=f: f(np.array(x, rt), np.array(y, rt)), _PAIRS, rng)
+
+    def div(x, y, rt):
+        if np.issubdtype(rt, np.floating):
+            return np.array(x, rt) / np.array(y, rt)
+        return _wrap_int(_cxx_int_div(x, y), rt)
+
+    def mod(x, y, rt):
+        return _wrap_int(int(x) - _cxx_int_div(x, y) * int(y), rt)
+
+    _check_binary("DIVIDES", div, _PAIRS, rng, b={"nonzero": True})
+    int_pairs = [p for p in _PAIRS if not any(np.issubdtype(t, np.floating) for t in p)]
+    _check_binary("MODULUS", mod, [p for p in int_pairs if np.bool_ not in p], rng, b={"nonzero": True})
+
+
+def test_compare_and_logical_opcodes_closed_form():
+    rng = np.random.default_rng(32)
+    boolr = lambda ta, tb: np.bool_  # noqa: E731
+    cmps = {"LESS": np.less, "LESS_EQUAL": np.less_equal, "GREATER": np.greater,
+            "GREATER_EQUAL": np.greater_equal, "EQUAL_TO": np.equal, "NOT_EQUAL_TO": np.not_equal}
+    for op, f in cmps.items():
+        # compared after the usual arithmetic conversions: a negative int is
+        # converted to unsigned against an unsigned operand
+        for ta, tb in _PAIRS:
+            a, b = _operands(ta, rng), _operands(tb, rng)
+            b[:8] = a[:8].astype(tb)  # some equal pairs
+            A_, B_ = host(a), host(b)
+            o = oracle_for(A_, B_)
+            e = core.Func(A.OP[op], A_.dev, B_.dev)
+            assert xt.dtype_name(o.info(e)[0]) == "bool"
+            ct = _usual(ta, tb)
+            assert np.array_equal(o.assign(e), f(a.astype(ct), b.astype(ct))), (op, ta, tb)
+    for op, f in (("LOGICAL_AND", np.logical_and), ("LOGICAL_OR", np.logical_or)):
+        _check_binary(op, lambda x, y, rt, f=f: f(x != 0, y != 0), _PAIRS, rng, result=boolr)
+    # -1 < 1u is false in C++ (int converts to unsigned)
+    m1, one = host(np.array([-1], np.int32)), host(np.array([1], np.uint32))
+    assert not oracle_for(m1, one).assign(m1.dev < one.dev)[0]
+
+
+def test_bitwise_and_shift_opcodes_closed_form():
+    rng = np.random.default_rng(33)
+    int_pairs = [(np.int8, np.uint8), (np.int32, np.uint32), (np.int16, np.int64), (np.uint64, np.int64),
+                 (np.bool_, np.int16), (np.uint16, np.int32)]
+    for op, f in (("BITWISE_AND", np.bitwise_and), ("BITWISE_OR", np.bitwise_or), ("BITWISE_XOR", np.bitwise_xor)):
+        _check_binary(op, lambda x, y, rt, f=f: f(np.array(x, rt), np.array(y, rt)), int_pairs, rng)
+    # shifts take the promoted type of the left operand only ([expr.shift])
+    lr = lambda ta, tb: _promote(ta)  # noqa: E731
+    _check_binary("LEFT_SHIFT", lambda x, y, rt: np.left_shift(np.array(x, rt), np.array(y, rt)),
+                  int_pairs, rng, result=lr, a={"shift": True}, b={"shift": True})
+    _check_binary("RIGHT_SHIFT", lambda x, y, rt: np.right_shift(np.array(x, rt), np.array(y, rt)),
+                  int_pairs, rng, result=lr, b={"shift": True})
+
+
+def test_unary_and_select_opcodes_closed_form():
+    rng = np.random.default_rng(34)
+    for t in (np.int8, np.uint8, np.int32, np.uint32, np.int64, np.float32, np.float64, np.bool_):
+        a = _operands(t, rng)
+        A_ = host(a)
+        o = oracle_for(A_)
+        pt = _promote(t)
+        assert np.array_equal(o.assign(-A_.dev), -(a.astype(pt))) if t != np.bool_ else True
+        assert np.array_equal(o.assign(+A_.dev), a.astype(pt))
+        assert xt.dtype_name(o.info(+A_.dev)[0]) == np.dtype(pt).name
+        assert np.array_equal(o.assign(xt.logical_not(A_.dev)), a == 0)
+        if not np.issubdtype(t, np.floating):
+            assert np.array_equal(o.assign(~A_.dev), ~(a.astype(pt)))
+    # where(c, a, b) -> common type of a and b; maximum / minimum select
+    for ta, tb in _PAIRS:
+        a, b = _operands(ta, rng), _operands(tb, rng)
+        c = rng.integers(0, 2, a.shape).astype(np.bool_)
+        A_, B_, C_ = host(a), host(b), host(c)
+        o = oracle_for(A_, B_, C_)
+        ct = np.result_type(np.array(0, ta), np.array(0, tb)) if False else _common(ta, tb)
+        assert np.array_equal(o.assign(xt.where(C_.dev, A_.dev, B_.dev)),
+                              np.where(c, a.astype(ct), b.astype(ct))), (ta, tb)
+        assert np.array_equal(o.assign(xt.maximum(A_.dev, B_.dev)),
+                              np.where(a.astype(ct) > b.astype(ct), a.astype(ct), b.astype(ct))), (ta, tb)
+        assert np.array_equal(o.assign(xt.minimum(A_.dev, B_.dev)),
+                              np.where(a.astype(ct) < b.astype(ct), a.astype(ct), b.astype(ct))), (ta, tb)
+    # static_cast: float -> int truncates toward zero; int64 -> int8 wraps; int -> float rounds
+    f = np.array([-3.75, -2.5, -0.5, 0.5, 2.5, 3.75, 100.9], np.float64)
+    F = host(f)
+    assert np.array_equal(oracle_for(F).assign(xt.cast(F.dev, "int32")), np.trunc(f).astype(np.int32))
+    w = np.array([127, 128, 255, 256, -129, 1 << 40], np.int64)
+    W = host(w)
+    assert np.array_equal(oracle_for(W).assign(xt.cast(W.dev, "int8")), w.astype(np.int8))
+    big = np.array([(1 << 24) + 1, (1 << 53) + 1], np.int64)
+    Bg = host(big)
+    assert np.array_equal(oracle_for(Bg).assign(xt.cast(Bg.dev, "float32")), big.astype(np.float32))
+
+
+def _common(ta, tb):
+    """std::common_type_t of two arithmetic types: the usual arithmetic
+    conversions, except that two operands of one type keep it (bool, int8)."""
+    ta, tb = np.dtype(ta).type, np.dtype(tb).type
+    return ta if ta == tb else _usual(ta, tb)
+
+
+def test_rounding_and_math_opcodes_closed_form():
+    q = np.array([-2.5, -1.5, -0.75, -0.5, -0.25, 0.0, 0.25, 0.5, 0.75, 1.5, 2.5, 3.5], np.float64)
+    Q = host(q)
+    o = oracle_for(Q)
+    assert np.array_equal(o.assign(xt.floor(Q.dev)), np.floor(q))
+    assert np.array_equal(o.assign(xt.ceil(Q.dev)), np.ceil(q))
+    assert np.array_equal(o.assign(xt.trunc(Q.dev)), np.trunc(q))
+    # std::round: halfway cases away from zero (numpy's round is to even)
+    assert np.array_equal(o.assign(xt.round(Q.dev)), np.sign(q) * np.floor(np.abs(q) + 0.5))
+    assert np.array_equal(o.assign(xt.fabs(Q.dev)), np.abs(q))
+    iu = host(np.array([0, 3, 250], np.uint8))
+    assert np.array_equal(oracle_for(iu).assign(xt.abs(iu.dev)), np.array([0, 3, 250]))  # identity on unsigned
+    ii = host(np.array([-7, 0, 9], np.int32))
+    assert np.array_equal(oracle_for(ii).assign(xt.abs(ii.dev)), np.array([7, 0, 9], np.int32))
+    rng = np.random.default_rng(35)
+    x = rng.uniform(0.05, 0.95, 200)
+    X = host(x)
+    o = oracle_for(X)
+    import math
+    # Python's math module calls the same platform libm the oracle's std::
+    # functions do, so every one-to-one mapping is pinned bit for bit;
+    # exp2 and cbrt (absent from math before Python 3.11) against numpy
+    # within 4 ulp (glibc's cbrt is up to 3 ulp off).  A different function,
+    # or an argument taken from the wrong operand, is off by far more.
+    from helpers import ulp_diff
+    libm = {"exp": math.exp, "expm1": math.expm1, "log": math.log, "log2": math.log2, "log10": math.log10,
+            "log1p": math.log1p, "sqrt": math.sqrt, "sin": math.sin, "cos": math.cos, "tan": math.tan,
+            "tanh": math.tanh, "sinh": math.sinh, "cosh": math.cosh, "asin": math.asin, "acos": math.acos,
+            "atan": math.atan, "erf": math.erf}
+    for name, f in libm.items():
+        got = o.assign(getattr(xt, name)(X.dev))
+        assert np.array_equal(got, np.array([f(v) for v in x])), name
+    for name, f in {"exp2": np.exp2, "cbrt": np.cbrt}.items():
+        assert ulp_diff(o.assign(getattr(xt, name)(X.dev)), f(x)).max() <= 4, name
+    y = rng.uniform(0.5, 3.0, 200)
+    Y = host(y)
+    o = oracle_for(X, Y)
+    for name, f in {"pow": math.pow, "atan2": math.atan2, "fmod": math.fmod}.items():
+        got = o.assign(getattr(xt, name)(X.dev, Y.dev))
+        assert np.array_equal(got, np.array([f(a, b) for a, b in zip(x, y)])), name
+    # math.hypot is CPython's own algorithm, not libm's: within 1 ulp
+    got = o.assign(xt.hypot(X.dev, Y.dev))
+    assert ulp_diff(got, np.array([math.hypot(a, b) for a, b in zip(x, y)])).max() <= 1
+    # std::exp(int) computes in double
+    n = host(np.array([0, 1, 2], np.int32))
+    assert xt.dtype_name(oracle_for(n).info(xt.exp(n.dev))[0]) == "float64"
+
+
+# ---------------------------------------------------------------- immediate reducers
+# test_xreducer.cpp:704-771: sum(a, axes, evaluation_strategy::immediate) equals
+# the lazy sum on arange data (exact in double) and on column-major shorts.
+IMMEDIATE_AXES_3 = [None, [1], [0, 2], [1, 2]]
+IMMEDIATE_AXES_5 = [None, [1], [0, 2], [1, 2], [1, 3, 4], [0, 1, 4], [0, 1, 3], [0, 2, 3], [1, 2, 3]]
+
+
+@pytest.mark.parametrize("shape,axes_list", [((3, 3, 3), IMMEDIATE_AXES_3), ((4, 3, 6, 2, 7), IMMEDIATE_AXES_5)])
+def test_immediate_equals_lazy_kat(shape, axes_list):
+    a = host(np.arange(np.prod(shape), dtype=np.float64).reshape(shape))
+    o = oracle_for(a)
+    for axes in axes_list:
+        lazy = o.reduce(xt.sum(a.dev, axes))
+        imm = o.reduce(xt.sum(a.dev, axes, evaluation_strategy="immediate"))
+        assert np.array_equal(lazy, imm), axes
+        assert np.array_equal(imm, np.sum(a.host, axis=None if axes is None else tuple(axes)))
+
+
+@pytest.mark.parametrize("shape,axes_list", [((1, 5, 3), [[0, 2]]), ((1, 5, 1, 2, 3), [[0, 1, 2], [2, 3], [1, 3]])])
+def test_immediate_equals_lazy_column_major_shorts(shape, axes_list):
+    rng = np.random.default_rng(41)
+    a = host(np.asfortranarray(rng.integers(-1000, 1000, shape).astype(np.int16)))
+    o = oracle_for(a)
+    for axes in axes_list:
+        lazy = o.reduce(xt.sum(a.dev, axes))
+        imm = o.reduce(xt.sum(a.dev, axes, evaluation_strategy="immediate"))
+        assert np.array_equal(lazy, imm), axes
+        assert np.array_equal(imm, np.sum(a.host.astype(np.int32), axis=tuple(axes)))
+
+
+def test_immediate_fold_order():
+    """reduce_immediate's order, checked against explicit loops in float32
+    (where order shows in the rounding):
+    * every axis: one std::accumulate over the storage (reducers/xreducer.hpp:354-360)
+      -- the lazy form folds each row and merges the rows instead;
+    * the leading axis strided (sum over axis 0): each output folded down the axis;
+    * the leading axis contiguous with an outer reduced axis ({0, 2} of a 3-d
+      array): each run accumulated from 0, then merged into its output;
+    * xt::initial merged behind the value; with no axes it is not applied."""
+    rng = np.random.default_rng(42)
+    x = rng.normal(size=(9, 300)).astype(np.float32) * np.float32(1e3)
+    X = host(x)
+    o = oracle_for(X)
+    acc = np.float32(0)
+    for v in x.reshape(-1):
+        acc = np.float32(acc + v)
+    imm = o.reduce(xt.sum(X.dev, None, evaluation_strategy="immediate"))[()]
+    assert imm == acc
+    lazy = o.reduce(xt.sum(X.dev))[()]
+    racc = np.float32(0)
+    for r in range(9):
+        row = np.float32(0)
+        for v in x[r]:
+            row = np.float32(row + v)
+        racc = np.float32(racc + row)
+    assert lazy == racc and lazy != imm  # the two orders differ on this input
+    col = o.reduce(xt.sum(X.dev, [0], evaluation_strategy="immediate"))
+    want = np.zeros(300, np.float32)
+    for r in range(9):
+        want = (want + x[r]).astype(np.float32)
+    assert np.array_equal(col, want)
+    y = rng.normal(size=(4, 5, 60)).astype(np.float32) * np.float32(1e3)
+    Y = host(y)
+    got = oracle_for(Y).reduce(xt.sum(Y.dev, [0, 2], evaluation_strategy="immediate"))
+    for j in range(5):
+        out = None
+        for i in range(4):
+            run = np.float32(0)
+            for v in y[i, j]:
+                run = np.float32(run + v)
+            out = run if out is None else np.float32(out + run)
+        assert got[j] == out
+    got = oracle_for(Y).reduce(xt.sum(Y.dev, [2], initial=np.float32(7), evaluation_strategy="immediate"))
+    lz = oracle_for(Y).reduce(xt.sum(Y.dev, [2], evaluation_strategy="immediate"))
+    assert np.array_equal(got, (lz + np.float32(7)).astype(np.float32))
+    n = host(np.array([1, 2, 3], np.int32))
+    assert np.array_equal(oracle_for(n).reduce(xt.sum(n.dev, [], initial=5, evaluation_strategy="immediate")),
+                          np.array([1, 2, 3]))

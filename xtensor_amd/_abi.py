@@ -48,6 +48,7 @@ OP = dict(
 RED = dict(sum=0, prod=1, amax=2, amin=3, mean=4, norm_sq=5, norm_l2=6, norm_l1=7, norm_linf=8, norm_l0=9,
            variance=10, stddev=11, minmax=12)
 SCAN = dict(cumsum=0, cumprod=1)
+REDUCE_IMMEDIATE = 1  # xg_reduce_desc.flags: xt::evaluation_strategy::immediate
 AXIS_NONE = -1000
 
 
@@ -190,6 +191,8 @@ DEBUG_EXPORTS = {
     "xg_debug_jit_stats": (C.c_char_p, [C.POINTER(C.c_int64)]),
     "xg_debug_assign_kernel": (C.c_int, [_P(Expr), _P(Tensor), C.c_char_p, C.c_int32]),
     "xg_debug_assign_pieces": (C.c_int, [_P(Expr), _P(Tensor), _P(C.c_int64), _P(C.c_int32), _P(C.c_int64)]),
+    "xg_debug_reduce_pieces": (C.c_int, [_P(Expr), _P(ReduceDesc), _P(Tensor), _P(C.c_int64), _P(C.c_int32),
+                                         _P(C.c_int64), _P(C.c_int32)]),
 }
 
 _lib = None

@@ -521,9 +521,15 @@ def cast(e, dtype) -> Func:
 class Reducer(Expr):
     """Lazy reduction; evaluated by `eval`, assignment, or use in an expression."""
 
-    def __init__(self, kind, e, axes=None, keep_dims=False, initial=None, dtype=None, ddof=0):
+    def __init__(self, kind, e, axes=None, keep_dims=False, initial=None, dtype=None, ddof=0,
+                 evaluation_strategy="lazy"):
         self.kind = kind
         self.ddof = int(ddof)
+        if evaluation_strategy not in ("lazy", "immediate"):
+            raise ValueError("evaluation_strategy is 'lazy' or 'immediate'")
+        # xt::evaluation_strategy::immediate (reducers/xreducer.hpp:944-952):
+        # the expression is evaluated first, then reduced
+        self.immediate = evaluation_strategy == "immediate"
         self.expr = _wrap(e)
         if axes is not None and isinstance(axes, (numbers.Integral, np.integer)):
             axes = [int(axes)]
@@ -549,6 +555,7 @@ class Reducer(Expr):
             d.has_initial = 1
             d.initial = self.initial.to_abi()
         d.ddof = self.ddof
+        d.flags = A.REDUCE_IMMEDIATE if self.immediate else 0
         return d
 
     def _info(self):
@@ -556,8 +563,8 @@ class Reducer(Expr):
 
 
 def _reducer(kind):
-    def f(e, axes=None, keep_dims=False, initial=None, dtype=None):
-        return Reducer(kind, e, axes, keep_dims, initial, dtype)
+    def f(e, axes=None, keep_dims=False, initial=None, dtype=None, evaluation_strategy="lazy"):
+        return Reducer(kind, e, axes, keep_dims, initial, dtype, evaluation_strategy=evaluation_strategy)
     f.__name__ = kind
     f.__doc__ = f"xt::{kind}(e, axes, options)"
     return f
@@ -567,15 +574,15 @@ for _k in A.RED:
     globals()[_k] = _reducer(_k)
 
 
-def variance(e, axes=None, ddof=0, keep_dims=False, dtype=None):
+def variance(e, axes=None, ddof=0, keep_dims=False, dtype=None, evaluation_strategy="lazy"):
     """xt::variance<T>(e, axes, ddof) (core/xmath.hpp:2031-2113): mean<T> of
     square(e - mean<T>(e, axes)), divided by (size - ddof) / outputs."""
-    return Reducer("variance", e, axes, keep_dims, None, dtype, ddof)
+    return Reducer("variance", e, axes, keep_dims, None, dtype, ddof, evaluation_strategy)
 
 
-def stddev(e, axes=None, keep_dims=False, dtype=None):
+def stddev(e, axes=None, keep_dims=False, dtype=None, evaluation_strategy="lazy"):
     """xt::stddev<T>(e, axes) = sqrt(variance<T>(e, axes)) (core/xmath.hpp:2046, 2143)."""
-    return Reducer("stddev", e, axes, keep_dims, None, dtype)
+    return Reducer("stddev", e, axes, keep_dims, None, dtype, evaluation_strategy=evaluation_strategy)
 
 
 def minmax(e):
