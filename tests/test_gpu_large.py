@@ -84,3 +84,69 @@ def test_reductions_over_a_dimension_past_2_31():
     assert abs(got - ref) / ref <= 2e-7, (got, ref)
     del f
     torch.cuda.empty_cache()
+
+
+def test_reductions_in_pieces_merge_every_kind():
+    """prod / amin / mean / norms through the merge of the pieces, xt::initial
+    applied once, keep_dims outputs, and reduced dimensions that only pass
+    2^31 elements once merged: exact against torch's integer arithmetic."""
+    import torch
+    n = (1 << 31) + (1 << 21) + 3
+    x = torch.randint(-100, 100, (n,), dtype=torch.int8, device="cuda")
+    X = wrap(x, (n,))
+    total = int(x.sum(dtype=torch.int64).item())
+    assert int(xt.eval(xt.amin(X)).to_numpy()) == int(x.min().item())
+    assert int(xt.eval(xt.sum(X, None, initial=5, dtype="int64")).to_numpy()) == total + 5
+    # mean<void>(int8): the sum in int, divided in double by the whole count
+    assert float(xt.eval(xt.mean(X)).to_numpy()) == total / n
+    assert int(xt.eval(xt.norm_l1(X)).to_numpy()) == int(x.to(torch.int64).abs().sum().item())
+    assert int(xt.eval(xt.norm_l0(X)).to_numpy()) == int((x != 0).sum().item())
+    assert int(xt.eval(xt.norm_linf(X)).to_numpy()) == int(x.to(torch.int16).abs().max().item())
+    sq = int((x.to(torch.int64) ** 2).sum().item())
+    assert int(xt.eval(xt.norm_sq(X)).to_numpy()) == sq
+    del x, X
+    # prod of +-1 values: the sign of the count of -1s, exact
+    s = torch.randint(0, 2, (n,), dtype=torch.int8, device="cuda") * 2 - 1
+    S = wrap(s, (n,))
+    neg = int((s < 0).sum().item())
+    assert int(xt.eval(xt.prod(S, None, dtype="int64")).to_numpy()) == (-1 if neg % 2 else 1)
+    assert int(xt.eval(xt.prod(S, None, initial=3, dtype="int64")).to_numpy()) == (-3 if neg % 2 else 3)
+    del s, S
+    # keep_dims over the long axis of [2, 2^31 + 5]
+    c = (1 << 31) + 5
+    x2 = torch.randint(-100, 100, (2, c), dtype=torch.int8, device="cuda")
+    got = xt.eval(xt.sum(wrap(x2, (2, c)), [1], keep_dims=True, dtype="int64")).to_numpy()
+    assert got.shape == (2, 1)
+    assert got[:, 0].tolist() == x2.sum(1, dtype=torch.int64).tolist()
+    got = xt.eval(xt.amin(wrap(x2, (2, c)), [1], keep_dims=True)).to_numpy()
+    assert got[:, 0].tolist() == x2.min(1).values.tolist()
+    del x2
+    # full reductions whose dimensions merge past 2^31: [2^31 + 5, 2] and [2, 2^30]
+    for shape in (((1 << 31) + 5, 2), (2, 1 << 30)):
+        x3 = torch.randint(-100, 100, shape, dtype=torch.int8, device="cuda")
+        got = int(xt.eval(xt.sum(wrap(x3, shape), None, dtype="int64")).to_numpy())
+        assert got == int(x3.sum(dtype=torch.int64).item()), shape
+        del x3
+    torch.cuda.empty_cache()
+
+
+def test_float_moments_and_norm_in_pieces():
+    """mean, norm_l2 and variance / stddev of 2^31 + 7 float32 values: the
+    divisor is the whole count, the square root follows the merge; within
+    stated bounds of torch's float64 evaluation."""
+    import torch
+    m = (1 << 31) + 7
+    f = torch.rand(m, dtype=torch.float32, device="cuda")
+    F = wrap(f, (m,))
+    fd = f.double()
+    mean_ref = float(fd.mean().item())
+    assert abs(float(xt.eval(xt.mean(F)).to_numpy()) - mean_ref) / mean_ref <= 2e-7
+    l2_ref = float(fd.norm().item())
+    assert abs(float(xt.eval(xt.norm_l2(F)).to_numpy()) - l2_ref) / l2_ref <= 2e-7
+    var_ref = float(fd.var(unbiased=False).item())
+    got = float(xt.eval(xt.variance(F)).to_numpy())
+    assert abs(got - var_ref) / var_ref <= 1e-6, (got, var_ref)
+    got = float(xt.eval(xt.stddev(F)).to_numpy())
+    assert abs(got - var_ref ** 0.5) / var_ref ** 0.5 <= 1e-6
+    del f, F, fd
+    torch.cuda.empty_cache()

@@ -416,3 +416,57 @@ def test_mean_float_accumulator():
     assert got.dtype == np.float32
     ref = oracle_for(a).reduce(r, high_precision=True)
     assert rel_err(got, ref) <= 2e-7
+
+
+# ---------------------------------------------------------------- evaluation_strategy::immediate
+IMMEDIATE_AXES = {(3, 3, 3): [None, [1], [0, 2], [1, 2]],
+                  (4, 3, 6, 2, 7): [None, [1], [0, 2], [1, 2], [1, 3, 4], [0, 1, 4], [0, 1, 3], [0, 2, 3],
+                                    [1, 2, 3]]}
+
+
+@pytest.mark.parametrize("shape", list(IMMEDIATE_AXES))
+def test_immediate_matches_lazy_kat(shape):
+    """test_xreducer.cpp:704-771 on the device: immediate == lazy == the
+    oracle's restatement of reduce_immediate, on arange data (exact)."""
+    h = np.arange(np.prod(shape), dtype=np.float64).reshape(shape)
+    a = Dual.of(h)
+    o = oracle_for(a)
+    for axes in IMMEDIATE_AXES[shape]:
+        lazy, _ = run(xt.sum(a.dev, axes))
+        imm, _ = run(xt.sum(a.dev, axes, evaluation_strategy="immediate"))
+        ref = o.reduce(xt.sum(a.dev, axes, evaluation_strategy="immediate"))
+        assert np.array_equal(lazy, imm) and np.array_equal(imm, ref), axes
+        # an expression is evaluated first, then reduced
+        imm2, _ = run(xt.sum(a.dev * 2.0 - 1.0, axes, evaluation_strategy="immediate"))
+        assert np.array_equal(imm2, o.reduce(xt.sum(a.dev * 2.0 - 1.0, axes, evaluation_strategy="immediate"))), axes
+
+
+def test_immediate_column_major_and_float_results():
+    """Column-major shorts (test_xreducer.cpp:765-770) exactly, and float32
+    data through an expression within the float bar of the long-double
+    reference, for lazy and immediate alike."""
+    rng = np.random.default_rng(51)
+    h = rng.integers(-1000, 1000, (3, 2, 1, 5, 1)).astype(np.int16)  # the transpose of [1,5,1,2,3]
+    base = Dual.of(h)
+    ct = base.transpose()  # shape [1, 5, 1, 2, 3], column-major strides
+    o = oracle_for(base, ct)
+    for axes in ([0, 1, 2], [2, 3], [1, 3]):
+        imm, _ = run(xt.sum(ct.dev, axes, evaluation_strategy="immediate"))
+        lazy, _ = run(xt.sum(ct.dev, axes))
+        ref = o.reduce(xt.sum(ct.dev, axes, evaluation_strategy="immediate"))
+        assert np.array_equal(imm, lazy) and np.array_equal(imm, ref), axes
+    x = rng.normal(1.0, 1.0, (64, 33, 129)).astype(np.float32)
+    X = Dual.of(x)
+    o = oracle_for(X)
+    for axes in (None, [0], [2], [0, 2], [1, 2]):
+        e = xt.sum(X.dev * X.dev + 1.0, axes, evaluation_strategy="immediate")
+        got, _ = run(e)
+        hp = o.reduce(e, high_precision=True)
+        assert rel_err(got, hp) <= 1e-12, axes  # double sums of float32 values
+        got, _ = run(xt.mean(X.dev, axes, evaluation_strategy="immediate"))
+        # mean<void>(float32): a float32 sum (compensated), then divided in double
+        assert rel_err(got, o.reduce(xt.mean(X.dev, axes), high_precision=True)) <= 2e-7, axes
+    # variance: both passes evaluate first
+    got, _ = run(xt.variance(X.dev, [1], evaluation_strategy="immediate"))
+    lazy, _ = run(xt.variance(X.dev, [1]))
+    assert rel_err(got, lazy) <= 1e-12

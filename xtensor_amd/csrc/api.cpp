@@ -100,6 +100,19 @@ void* workspace(size_t bytes, hipStream_t stream) {
     return b.ptr;
 }
 
+// base address of a sizing pass's block: 256-byte aligned like a real one,
+// so that plans made against it choose the same vector accesses
+constexpr uintptr_t kSizingBase = uintptr_t(1) << 40;
+
+char* Arena::take(size_t bytes) {
+    if (bytes == 0) return nullptr;
+    const size_t off = (used + 255) & ~size_t(255);
+    used = off + bytes;
+    if (measure) return reinterpret_cast<char*>(kSizingBase) + off;
+    if (used > cap) raise(XG_INVALID_ARGUMENT, "internal: a call took more device memory than its sizing pass");
+    return base + off;
+}
+
 }  // namespace xg
 
 using namespace xg;
@@ -131,129 +144,6 @@ RedPlan xg::plan_reduce_best(const xg_expr* e, const xg_reduce_desc* r, const xg
 
 namespace {
 
-// xt::variance / xt::stddev (core/xmath.hpp:2031-2113): the inner
-// mean<T>(e, axes) is evaluated first into a kept-dims temporary m, then
-// mean<T>(square(e - m), axes, ddof) -- two passes over e, as in xtensor.
-// m and the split partials share the stream's workspace.
-void reduce_moment(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t st) {
-    if (!e || !e->nodes || e->n_nodes < 1) raise(XG_INVALID_ARGUMENT, "empty expression");
-    xg_reduce_desc r1 = *r;
-    r1.op = XG_RED_MEAN;
-    r1.keep_dims = 1;
-    r1.has_initial = 0;
-    r1.ddof = 0;
-    int mdt = 0;
-    Shape ms;
-    reduce_result(e, &r1, &mdt, &ms);
-    const int64_t nm = numel(ms);
-    const size_t mbytes = (size_t(std::max<int64_t>(nm, 1)) * size_t(dtype_size(mdt)) + 255) & ~size_t(255);
-    xg_tensor m{};
-    m.dtype = mdt;
-    m.ndim = int32_t(ms.size());
-    int64_t stv = 1;
-    for (int k = m.ndim - 1; k >= 0; --k) {
-        m.shape[k] = ms[size_t(k)];
-        m.strides[k] = stv;
-        stv *= std::max<int64_t>(ms[size_t(k)], 1);
-    }
-    // e - m: the tree of e, m as one more leaf, and the subtraction as root
-    std::vector<xg_node> nodes(e->nodes, e->nodes + e->n_nodes);
-    std::vector<xg_tensor> leaves(e->leaves, e->leaves + e->n_leaves);
-    leaves.push_back(m);
-    xg_node ln{};
-    ln.op = XG_OP_LEAF;
-    ln.index = e->n_leaves;
-    nodes.push_back(ln);
-    xg_node sub{};
-    sub.op = XG_OP_MINUS;
-    sub.nargs = 2;
-    sub.args[0] = e->n_nodes - 1;
-    sub.args[1] = e->n_nodes;
-    nodes.push_back(sub);
-    xg_expr e2 = *e;
-    e2.n_nodes = int32_t(nodes.size());
-    e2.n_leaves = int32_t(leaves.size());
-    e2.nodes = nodes.data();
-    e2.leaves = leaves.data();
-    // plan with a placeholder address to size the workspace, then for real
-    auto plan_both = [&](char* base, RedPlan* p1, int* V1, RedPlan* p2, int* V2) {
-        m.data = base;
-        leaves.back().data = base;
-        *p1 = plan_reduce_best(e, &r1, &m, V1);
-        *p2 = plan_reduce_best(&e2, r, out, V2);
-    };
-    RedPlan p1, p2;
-    int V1 = 4, V2 = 4;
-    plan_both(reinterpret_cast<char*>(uintptr_t(1) << 40), &p1, &V1, &p2, &V2);
-    if (p2.n_out == 0) {
-        note_kernel("empty");
-        return;
-    }
-    const size_t wbytes = std::max(reduce_workspace_bytes(p1), reduce_workspace_bytes(p2));
-    char* base = static_cast<char*>(workspace(mbytes + wbytes, st));
-    plan_both(base, &p1, &V1, &p2, &V2);
-    const char* used = nullptr;
-    if (p1.n_out) {
-        p1.partials = wbytes ? base + mbytes : nullptr;
-        check_hip(launch_reduce(p1, V1, st, &used), "variance mean launch");
-    }
-    p2.partials = wbytes ? base + mbytes : nullptr;
-    check_hip(launch_reduce(p2, V2, st, &used), "variance launch");
-    note_kernel(used);
-}
-
-// xt::minmax (core/xmath.hpp:2195-2230): out[0] = the std::min fold,
-// out[1] = the std::max fold over every element, from max() / lowest().
-void reduce_minmax(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t st) {
-    int dt = 0;
-    Shape rs;
-    reduce_result(e, r, &dt, &rs);
-    if (!out || out->ndim != 1 || out->shape[0] != 2)
-        raise(XG_BROADCAST_ERROR, "minmax output must have shape (2)");
-    const char* used = nullptr;
-    for (int k = 0; k < 2; ++k) {
-        xg_reduce_desc rk{};
-        rk.op = k == 0 ? kRedMinNum : kRedMaxNum;
-        rk.acc_dtype = -1;
-        rk.n_axes = -1;
-        xg_tensor ok = *out;
-        ok.ndim = 0;
-        ok.offset = out->offset + int64_t(k) * out->strides[0];
-        int V = 4;
-        RedPlan p = plan_reduce_best(e, &rk, &ok, &V);
-        p.partials = workspace(reduce_workspace_bytes(p), st);
-        check_hip(launch_reduce(p, V, st, &used), "minmax launch");
-    }
-    note_kernel(used);
-}
-
-}  // namespace
-
-extern "C" {
-
-const char* xg_version(void) { return "xtensor_amd 0.1.0 (gfx950)"; }
-
-const char* xg_last_error_message(void) { return t_err.c_str(); }
-
-// Name of the kernel family the calling thread's last successful entry point
-// used ("dynamic", "spec:<name>", "reduce_rows", ...).  Diagnostic only.
-const char* xg_last_kernel(void) { return t_last_kernel.c_str(); }
-
-xg_status xg_init(void) {
-    return guard([&] { device_cus(); });
-}
-
-xg_status xg_expr_info(const xg_expr* e, int32_t* dtype, int32_t* ndim, int64_t* shape) {
-    return guard([&] {
-        if (!dtype || !ndim || !shape) raise(XG_INVALID_ARGUMENT, "null output pointer");
-        ExprInfo info = analyse(e);
-        *dtype = info.dtype;
-        *ndim = int32_t(info.shape.size());
-        for (size_t k = 0; k < info.shape.size(); ++k) shape[k] = info.shape[k];
-    });
-}
-
-namespace {
 // An assignment whose offsets need 64 bits, cut along the output's first
 // dimension into pieces whose offsets fit 32 bits, so that every piece runs
 // a compiled (or run-time compiled) kernel instead of the generic
@@ -286,8 +176,14 @@ bool plan_in_pieces(const xg_expr* e, const xg_tensor* out, std::vector<Piece>* 
     }
     constexpr int64_t kLim = (int64_t(1) << 31) - 1;
     if (rest >= kLim) return false;
+    // rows per piece: offsets stay below 2^31, and so does every merged
+    // dimension, since a piece holds at most max(2^30, one row) elements
+    // (contiguous rows merge into one dimension of the piece's element count)
+    int64_t row_elems = 1;
+    for (int k = 1; k < out->ndim; ++k) row_elems *= std::max<int64_t>(out->shape[k], 1);
+    if (row_elems >= kLim) return false;
     int64_t rows = step0 > 0 ? (kLim - rest) / step0 + 1 : out->shape[0];
-    rows = std::min<int64_t>(rows, int64_t(1) << 30);  // and the inner extent stays below 2^31
+    rows = std::min<int64_t>(rows, std::max<int64_t>(1, (int64_t(1) << 30) / row_elems));
     if (rows >= out->shape[0]) return false;
     std::vector<xg_tensor> leaves(e->leaves, e->leaves + e->n_leaves);
     xg_expr pe = *e;
@@ -311,37 +207,394 @@ bool plan_in_pieces(const xg_expr* e, const xg_tensor* out, std::vector<Piece>* 
     }
     return true;
 }
+// out = e: one plan, or pieces along the first dimension when the offsets
+// need 64 bits or the iteration space does not fit one plan.
+void assign_launch(const xg_expr* e, const xg_tensor* out, hipStream_t st, const char** used) {
+    int V = 4;
+    EwPlan p{};
+    bool whole = true;  // false: the iteration space does not fit one plan's 32-bit unit indexing
+    Error refused{};
+    try {
+        p = plan_assign_best(e, out, &V);
+    } catch (const Error& err) {
+        if (err.code != XG_UNSUPPORTED) throw;
+        whole = false;
+        refused = err;
+    }
+    std::vector<Piece> pieces;
+    if ((!whole || p.index64) && plan_in_pieces(e, out, &pieces)) {
+        for (const Piece& pc : pieces)
+            check_hip(launch_ew(pc.first, pc.second, device_cus(), st, used), "elementwise launch");
+        return;
+    }
+    if (!whole) throw refused;
+    if (p.units == 0) return;
+    check_hip(launch_ew(p, V, device_cus(), st, used), "elementwise launch");
+}
+
+// Launch a planned reduction with its split partials from the arena.
+void run_reduce(RedPlan& p, int V, hipStream_t st, Arena& a, const char** used) {
+    p.partials = a.take(reduce_workspace_bytes(p));
+    if (!a.measure && p.n_out) check_hip(launch_reduce(p, V, st, used), "reduction launch");
+}
+
+// A reduction whose reduced extent does not fit 32-bit indexing (a reduced
+// dimension of 2^31 elements or more, or reduced dimensions that merge into
+// one that long), cut into pieces.  The innermost reduced axis at which the
+// product of the reduced extents from there inward passes 2^30 is cut into
+// chunks, and every reduced axis outside it is taken one index at a time,
+// so no piece reduces more than 2^30 elements per output.  Each piece folds
+// into its own slice of a temporary of the output's shape, in the folded
+// value type (a mean's sum, a norm_l2's sum of squares, a variance's sum of
+// squared deviations), and a second reduction over the pieces merges them
+// and finishes the result once: xt::initial, the divisor of the whole
+// reduction for mean / variance / stddev, the square root of norm_l2 and
+// stddev, the store conversion.
+struct ReducePieces {
+    bool ok = false;  // false: no reduced extent is too long, or a gather
+    int piece_op = 0, merge_op = 0, merge_acc = -1;
+    int R = 0, j = -1, d = -1;  // ax[j] = d is the axis cut into chunks
+    std::vector<int> ax;
+    Shape shape;
+    int64_t len_max = 0, chunks = 0, k_pieces = 0, out_elems = 0;
+    xg_reduce_desc pr{};           // the pieces' reduction
+    xg_tensor o{};                 // a piece's slice of the temporary
+    std::vector<xg_tensor> leaves;
+    xg_expr pe{};                  // the expression with the sliced leaves
+    const xg_expr* e = nullptr;
+
+    ReducePieces(const xg_expr* e_, const xg_reduce_desc* r, const xg_tensor* out, int vdt) : e(e_) {
+        if (!r || !out) return;
+        piece_op = merge_op = r->op;
+        merge_acc = r->acc_dtype;
+        switch (r->op) {
+        case XG_RED_SUM: case XG_RED_PROD: case XG_RED_AMAX: case XG_RED_AMIN: break;
+        case kRedMinNum: case kRedMaxNum: merge_acc = -1; break;
+        case XG_RED_MEAN: piece_op = XG_RED_SUM; merge_op = XG_RED_SUM; break;
+        case XG_RED_VARIANCE: case XG_RED_STDDEV: piece_op = XG_RED_VARIANCE; merge_op = XG_RED_SUM; break;
+        case XG_RED_NORM_SQ: case XG_RED_NORM_L1: case XG_RED_NORM_L0: merge_op = XG_RED_SUM; merge_acc = -1; break;
+        case XG_RED_NORM_L2: piece_op = XG_RED_NORM_SQ; merge_op = XG_RED_SUM; merge_acc = -1; break;
+        case XG_RED_NORM_LINF: merge_op = XG_RED_AMAX; merge_acc = -1; break;
+        default: return;
+        }
+        for (int i = 0; i < e->n_nodes; ++i)
+            if (e->nodes[i].op == XG_OP_GATHER) return;
+        shape = analyse(e).shape;
+        R = int(shape.size());
+        if (r->n_axes < 0) {  // the reduced axes, ascending (plan_reduce has checked them)
+            for (int k = 0; k < R; ++k) ax.push_back(k);
+        } else {
+            for (int i = 0; i < r->n_axes; ++i) ax.push_back(r->axes[i] < 0 ? r->axes[i] + R : r->axes[i]);
+        }
+        constexpr int64_t kPiece = int64_t(1) << 30;
+        constexpr int64_t kMaxPieces = 4096;
+        int64_t inner = 1;  // product of the reduced extents inside ax[j]
+        for (int i = int(ax.size()) - 1; i >= 0; --i) {
+            const int64_t n = shape[size_t(ax[size_t(i)])];
+            if (n > kPiece / inner) {
+                j = i;
+                break;
+            }
+            inner *= n;
+        }
+        if (j < 0) return;
+        d = ax[size_t(j)];
+        len_max = std::max<int64_t>(1, kPiece / inner);
+        chunks = (shape[size_t(d)] + len_max - 1) / len_max;
+        k_pieces = chunks;
+        for (int i = 0; i < j && k_pieces <= kMaxPieces; ++i) k_pieces *= shape[size_t(ax[size_t(i)])];
+        if (k_pieces > kMaxPieces) raise(XG_UNSUPPORTED, "reduction too large: it would need more than 4096 pieces");
+        out_elems = 1;
+        for (int k = 0; k < out->ndim; ++k) out_elems *= out->shape[k];
+        if (out_elems <= 0) return;
+        pr = *r;
+        pr.op = piece_op;
+        pr.has_initial = 0;
+        pr.ddof = 0;
+        o = *out;
+        o.dtype = vdt;
+        for (int64_t k = o.ndim - 1, stride = 1; k >= 0; --k) {
+            o.strides[k] = stride;
+            stride *= o.shape[k];
+        }
+        leaves.assign(e->leaves, e->leaves + e->n_leaves);
+        pe = *e;
+        pe.leaves = leaves.data();
+        ok = true;
+    }
+
+    // slice the leaves and the temporary for piece q
+    void select(int64_t q, char* tmp) {
+        int64_t rest = q / chunks;
+        const int64_t s0 = (q % chunks) * len_max, len = std::min(len_max, shape[size_t(d)] - s0);
+        std::vector<int64_t> idx(size_t(j), 0);  // indices of the reduced axes outside ax[j]
+        for (int i = j - 1; i >= 0; --i) {
+            const int64_t ext = shape[size_t(ax[size_t(i)])];
+            idx[size_t(i)] = rest % ext;
+            rest /= ext;
+        }
+        for (int l = 0; l < e->n_leaves; ++l) {
+            const xg_tensor& t = e->leaves[l];
+            xg_tensor& pl = leaves[size_t(l)];
+            pl = t;
+            auto cut = [&](int axis, int64_t start, int64_t extent) {
+                const int dl = axis - (R - t.ndim);  // the leaf's axis aligned with `axis`
+                if (dl >= 0 && t.shape[dl] != 1) {
+                    pl.offset += start * t.strides[dl];
+                    pl.shape[dl] = extent;
+                }
+            };
+            for (int i = 0; i < j; ++i) cut(ax[size_t(i)], idx[size_t(i)], 1);
+            cut(d, s0, len);
+        }
+        o.data = tmp;
+        o.offset = q * out_elems;
+    }
+};
+
+// Evaluate a reduction in pieces (see ReducePieces).  Returns false, doing
+// nothing, when no cut applies.
+bool reduce_in_pieces(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t st, Arena& a,
+                      const char** used) {
+    if (!r || r->op == XG_RED_MINMAX) return false;
+    const RedFinish fin = reduce_finish(e, r);
+    const int vdt = fin.vdt;
+    ReducePieces P(e, r, out, vdt);
+    if (!P.ok) return false;
+    char* tmp = a.take(size_t(P.k_pieces) * size_t(P.out_elems) * size_t(dtype_size(vdt)));
+    auto plan_piece = [&](int64_t q, int* V) {
+        P.select(q, tmp);
+        RedPlan p = plan_reduce_best(&P.pe, &P.pr, &P.o, V);
+        if (p.vdt != vdt) raise(XG_UNSUPPORTED, "reduction in pieces: value types of the pieces differ");
+        p.fin = FZ_NONE;  // the folded value only; the merge finishes
+        p.rdt = p.vdt;
+        return p;
+    };
+    // the merge: the pieces as [pieces, out shape...] reduced over axis 0
+    xg_tensor parts{};
+    parts.data = tmp;
+    parts.dtype = vdt;
+    parts.ndim = out->ndim + 1;
+    parts.shape[0] = P.k_pieces;
+    parts.strides[0] = P.out_elems;
+    for (int k = 0; k < out->ndim; ++k) {
+        parts.shape[k + 1] = P.o.shape[k];
+        parts.strides[k + 1] = P.o.strides[k];
+    }
+    xg_node leaf{};
+    leaf.op = XG_OP_LEAF;
+    xg_expr me{};
+    me.n_nodes = 1;
+    me.n_leaves = 1;
+    me.nodes = &leaf;
+    me.leaves = &parts;
+    xg_reduce_desc mr{};
+    mr.op = P.merge_op;
+    mr.acc_dtype = P.merge_acc;
+    mr.n_axes = 1;
+    mr.axes[0] = 0;
+    mr.has_initial = r->has_initial;
+    mr.initial = r->initial;
+    int Vm = 4;
+    RedPlan pm = plan_reduce_best(&me, &mr, out, &Vm);
+    if (pm.vdt != vdt) raise(XG_UNSUPPORTED, "reduction in pieces: the merge's value type differs");
+    pm.fin = fin.fin;
+    pm.mdt = fin.mdt;
+    pm.div_bits = fin.div_bits;
+    pm.rdt = fin.rdt;
+    // split partials: one region that every piece and the merge reuse in stream order
+    size_t pbytes = reduce_workspace_bytes(pm);
+    int V = 4;
+    for (int64_t q = 0; q < P.k_pieces; ++q) pbytes = std::max(pbytes, reduce_workspace_bytes(plan_piece(q, &V)));
+    char* part = a.take(pbytes);
+    if (a.measure) return true;
+    for (int64_t q = 0; q < P.k_pieces; ++q) {
+        RedPlan p = plan_piece(q, &V);
+        p.partials = reduce_workspace_bytes(p) ? part : nullptr;
+        check_hip(launch_reduce(p, V, st, used), "reduction launch");
+    }
+    pm.partials = reduce_workspace_bytes(pm) ? part : nullptr;
+    check_hip(launch_reduce(pm, Vm, st, used), "reduction launch");
+    return true;
+}
+
+// A reduction plan_reduce handles (every op but minmax; variance / stddev of
+// the deviation expression), in pieces when its reduced extent is too long.
+void reduce_plain(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t st, Arena& a,
+                  const char** used) {
+    int V = 4;
+    RedPlan p{};
+    try {
+        p = plan_reduce_best(e, r, out, &V);
+    } catch (const Error& err) {
+        if (err.code != XG_UNSUPPORTED || !reduce_in_pieces(e, r, out, st, a, used)) throw;
+        return;
+    }
+    if (p.n_out == 0) return;
+    run_reduce(p, V, st, a, used);
+}
+
+// xt::variance / xt::stddev (core/xmath.hpp:2031-2113): the inner
+// mean<T>(e, axes) is evaluated first into a kept-dims temporary m, then
+// mean<T>(square(e - m), axes, ddof) -- two passes over e, as in xtensor.
+void reduce_moment(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t st, Arena& a,
+                   const char** used) {
+    if (!e || !e->nodes || e->n_nodes < 1) raise(XG_INVALID_ARGUMENT, "empty expression");
+    xg_reduce_desc r1 = *r;
+    r1.op = XG_RED_MEAN;
+    r1.keep_dims = 1;
+    r1.has_initial = 0;
+    r1.ddof = 0;
+    int mdt = 0;
+    Shape ms;
+    reduce_result(e, &r1, &mdt, &ms);
+    const int64_t nm = numel(ms);
+    xg_tensor m{};
+    m.dtype = mdt;
+    m.ndim = int32_t(ms.size());
+    int64_t stv = 1;
+    for (int k = m.ndim - 1; k >= 0; --k) {
+        m.shape[k] = ms[size_t(k)];
+        m.strides[k] = stv;
+        stv *= std::max<int64_t>(ms[size_t(k)], 1);
+    }
+    m.data = a.take(size_t(std::max<int64_t>(nm, 1)) * size_t(dtype_size(mdt)));
+    // e - m: the tree of e, m as one more leaf, and the subtraction as root
+    std::vector<xg_node> nodes(e->nodes, e->nodes + e->n_nodes);
+    std::vector<xg_tensor> leaves(e->leaves, e->leaves + e->n_leaves);
+    leaves.push_back(m);
+    xg_node ln{};
+    ln.op = XG_OP_LEAF;
+    ln.index = e->n_leaves;
+    nodes.push_back(ln);
+    xg_node sub{};
+    sub.op = XG_OP_MINUS;
+    sub.nargs = 2;
+    sub.args[0] = e->n_nodes - 1;
+    sub.args[1] = e->n_nodes;
+    nodes.push_back(sub);
+    xg_expr e2 = *e;
+    e2.n_nodes = int32_t(nodes.size());
+    e2.n_leaves = int32_t(leaves.size());
+    e2.nodes = nodes.data();
+    e2.leaves = leaves.data();
+    reduce_plain(e, &r1, &m, st, a, used);
+    reduce_plain(&e2, r, out, st, a, used);
+}
+
+// xt::minmax (core/xmath.hpp:2195-2230): out[0] = the std::min fold,
+// out[1] = the std::max fold over every element, from max() / lowest().
+void reduce_minmax(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t st, Arena& a,
+                   const char** used) {
+    int dt = 0;
+    Shape rs;
+    reduce_result(e, r, &dt, &rs);
+    if (!out || out->ndim != 1 || out->shape[0] != 2)
+        raise(XG_BROADCAST_ERROR, "minmax output must have shape (2)");
+    for (int k = 0; k < 2; ++k) {
+        xg_reduce_desc rk{};
+        rk.op = k == 0 ? kRedMinNum : kRedMaxNum;
+        rk.acc_dtype = -1;
+        rk.n_axes = -1;
+        xg_tensor ok = *out;
+        ok.ndim = 0;
+        ok.offset = out->offset + int64_t(k) * out->strides[0];
+        reduce_plain(e, &rk, &ok, st, a, used);
+    }
+}
+
+void reduce_any(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t st, Arena& a,
+                const char** used) {
+    if (r && (r->op == XG_RED_VARIANCE || r->op == XG_RED_STDDEV)) reduce_moment(e, r, out, st, a, used);
+    else if (r && r->op == XG_RED_MINMAX) reduce_minmax(e, r, out, st, a, used);
+    else reduce_plain(e, r, out, st, a, used);
+}
+
+char* stream_workspace(size_t bytes, hipStream_t st, void*) { return static_cast<char*>(workspace(bytes, st)); }
+
 }  // namespace
+
+namespace {
+// xt::evaluation_strategy::immediate (reducers/xreducer.hpp:944-952):
+// reduce_immediate(f, eval(e), ...).  An expression is evaluated into a
+// row-major temporary of its value type first; a lone container is reduced
+// as it is (xt::eval returns it unchanged).
+void reduce_top(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t st, Arena& a,
+                const char** used) {
+    if (!r) raise(XG_INVALID_ARGUMENT, "null reduce descriptor");
+    if (r->flags & ~XG_REDUCE_IMMEDIATE) raise(XG_INVALID_ARGUMENT, "unknown reduction flags");
+    const bool lone_leaf = e && e->n_nodes == 1 && e->nodes && e->nodes[0].op == XG_OP_LEAF;
+    if (!(r->flags & XG_REDUCE_IMMEDIATE) || lone_leaf) {
+        reduce_any(e, r, out, st, a, used);
+        return;
+    }
+    const ExprInfo info = analyse(e);
+    xg_tensor t{};
+    t.dtype = info.dtype;
+    t.ndim = int32_t(info.shape.size());
+    int64_t stv = 1;
+    for (int k = t.ndim - 1; k >= 0; --k) {
+        t.shape[k] = info.shape[size_t(k)];
+        t.strides[k] = stv;
+        stv *= std::max<int64_t>(info.shape[size_t(k)], 1);
+    }
+    const int64_t n = numel(info.shape);
+    t.data = a.take(size_t(std::max<int64_t>(n, 0)) * size_t(dtype_size(t.dtype)));
+    if (!a.measure && n > 0) assign_launch(e, &t, st, used);
+    xg_node leaf{};
+    leaf.op = XG_OP_LEAF;
+    xg_expr te{};
+    te.n_nodes = 1;
+    te.n_leaves = 1;
+    te.nodes = &leaf;
+    te.leaves = &t;
+    reduce_any(&te, r, out, st, a, used);
+}
+}  // namespace
+
+void xg::reduce_call(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t st, BlockSource src,
+                     void* ctx) {
+    const char* used = nullptr;
+    Arena sizing;
+    reduce_top(e, r, out, st, sizing, &used);
+    Arena a;
+    a.measure = false;
+    a.cap = sizing.used;
+    a.base = sizing.used ? src(sizing.used, st, ctx) : nullptr;
+    used = nullptr;
+    reduce_top(e, r, out, st, a, &used);
+    note_kernel(used ? used : "empty");
+}
+
+extern "C" {
+
+const char* xg_version(void) { return "xtensor_amd 0.1.0 (gfx950)"; }
+
+const char* xg_last_error_message(void) { return t_err.c_str(); }
+
+// Name of the kernel family the calling thread's last successful entry point
+// used ("dynamic", "spec:<name>", "reduce_rows", ...).  Diagnostic only.
+const char* xg_last_kernel(void) { return t_last_kernel.c_str(); }
+
+xg_status xg_init(void) {
+    return guard([&] { device_cus(); });
+}
+
+xg_status xg_expr_info(const xg_expr* e, int32_t* dtype, int32_t* ndim, int64_t* shape) {
+    return guard([&] {
+        if (!dtype || !ndim || !shape) raise(XG_INVALID_ARGUMENT, "null output pointer");
+        ExprInfo info = analyse(e);
+        *dtype = info.dtype;
+        *ndim = int32_t(info.shape.size());
+        for (size_t k = 0; k < info.shape.size(); ++k) shape[k] = info.shape[k];
+    });
+}
 
 xg_status xg_assign(const xg_expr* e, const xg_tensor* out, xg_stream_t stream) {
     return guard([&] {
-        int V = 4;
-        EwPlan p{};
-        bool whole = true;  // false: the iteration space does not fit one plan's 32-bit unit indexing
-        Error refused{};
-        try {
-            p = plan_assign_best(e, out, &V);
-        } catch (const Error& err) {
-            if (err.code != XG_UNSUPPORTED) throw;
-            whole = false;
-            refused = err;
-        }
         const char* used = nullptr;
-        std::vector<Piece> pieces;
-        if ((!whole || p.index64) && plan_in_pieces(e, out, &pieces)) {
-            for (const Piece& pc : pieces)
-                check_hip(launch_ew(pc.first, pc.second, device_cus(), reinterpret_cast<hipStream_t>(stream), &used),
-                          "elementwise launch");
-            note_kernel(used ? used : "empty");
-            return;
-        }
-        if (!whole) throw refused;
-        if (p.units == 0) {
-            note_kernel("empty");
-            return;
-        }
-        check_hip(launch_ew(p, V, device_cus(), reinterpret_cast<hipStream_t>(stream), &used), "elementwise launch");
-        note_kernel(used);
+        assign_launch(e, out, reinterpret_cast<hipStream_t>(stream), &used);
+        note_kernel(used ? used : "empty");
     });
 }
 
@@ -357,143 +610,8 @@ xg_status xg_reduce_info(const xg_expr* e, const xg_reduce_desc* r, int32_t* dty
     });
 }
 
-namespace {
-// sum / prod / amax / amin over a reduced dimension of 2^31 elements or more
-// (beyond 32-bit indexing): that dimension is cut into pieces of 2^30, each
-// piece reduces into its own slice of a temporary of out's shape (in the
-// reduction's result type), and a second reduction of the same kind over
-// the pieces gives out (with xt::initial applied there, once).  Returns false, doing nothing, for other
-// reductions, gathers, or a second dimension that is itself too long.
-bool reduce_in_pieces(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t st) {
-    if (!r || !out || (r->op != XG_RED_SUM && r->op != XG_RED_PROD && r->op != XG_RED_AMAX && r->op != XG_RED_AMIN))
-        return false;
-    for (int i = 0; i < e->n_nodes; ++i)
-        if (e->nodes[i].op == XG_OP_GATHER) return false;
-    const ExprInfo info = analyse(e);
-    const int R = int(info.shape.size());
-    constexpr int64_t kBig = int64_t(1) << 31;
-    std::vector<bool> red(size_t(R), r->n_axes < 0);
-    for (int i = 0; i < r->n_axes; ++i) {
-        const int a = r->axes[i] < 0 ? r->axes[i] + R : r->axes[i];
-        if (a < 0 || a >= R) return false;
-        red[size_t(a)] = true;
-    }
-    int d = -1;
-    for (int k = 0; k < R; ++k) {
-        if (info.shape[size_t(k)] < kBig) continue;
-        if (!red[size_t(k)] || d >= 0) return false;
-        d = k;
-    }
-    if (d < 0) return false;
-    int64_t out_elems = 1;
-    for (int k = 0; k < out->ndim; ++k) out_elems *= out->shape[k];
-    if (out_elems <= 0) return false;
-    const int64_t n = info.shape[size_t(d)], piece = int64_t(1) << 30, k_pieces = (n + piece - 1) / piece;
-    // the pieces keep the reduction's own result type (out's may be narrower: cast on the final store)
-    int rdt = 0;
-    Shape rshape;
-    reduce_result(e, r, &rdt, &rshape);
-    const int esz = dtype_size(rdt);
-    void* tmp = nullptr;
-    if (hipMalloc(&tmp, size_t(k_pieces * out_elems * esz)) != hipSuccess)
-        raise(XG_OUT_OF_MEMORY, "temporary for a reduction in pieces");
-    try {
-        std::vector<xg_tensor> leaves(e->leaves, e->leaves + e->n_leaves);
-        xg_expr pe = *e;
-        pe.leaves = leaves.data();
-        xg_reduce_desc pr = *r;
-        pr.has_initial = 0;
-        xg_tensor o = *out;  // one slice of the temporary, contiguous, of out's shape
-        o.data = tmp;
-        o.dtype = rdt;
-        for (int64_t k = o.ndim - 1, stride = 1; k >= 0; --k) {
-            o.strides[k] = stride;
-            stride *= o.shape[k];
-        }
-        const char* used = nullptr;
-        for (int64_t i = 0; i < k_pieces; ++i) {
-            const int64_t s0 = i * piece, len = std::min(piece, n - s0);
-            for (int l = 0; l < e->n_leaves; ++l) {
-                const xg_tensor& t = e->leaves[l];
-                leaves[size_t(l)] = t;
-                const int dl = d - (R - t.ndim);  // the leaf's axis aligned with d
-                if (dl >= 0 && t.shape[dl] != 1) {
-                    leaves[size_t(l)].offset += s0 * t.strides[dl];
-                    leaves[size_t(l)].shape[dl] = len;
-                }
-            }
-            o.offset = i * out_elems;
-            int V = 4;
-            RedPlan p = plan_reduce_best(&pe, &pr, &o, &V);
-            p.partials = workspace(reduce_workspace_bytes(p), st);
-            check_hip(launch_reduce(p, V, st, &used), "reduction launch");
-        }
-        // the pieces' results as [pieces, out shape...], reduced over axis 0 into out
-        xg_tensor parts{};
-        parts.data = tmp;
-        parts.dtype = rdt;
-        parts.ndim = out->ndim + 1;
-        parts.shape[0] = k_pieces;
-        parts.strides[0] = out_elems;
-        for (int k = 0; k < out->ndim; ++k) {
-            parts.shape[k + 1] = o.shape[k];
-            parts.strides[k + 1] = o.strides[k];
-        }
-        xg_node leaf{};
-        leaf.op = XG_OP_LEAF;
-        xg_expr me{};
-        me.n_nodes = 1;
-        me.n_leaves = 1;
-        me.nodes = &leaf;
-        me.leaves = &parts;
-        xg_reduce_desc mr = *r;
-        mr.n_axes = 1;
-        mr.axes[0] = 0;
-        mr.keep_dims = 0;
-        int V = 4;
-        RedPlan p = plan_reduce_best(&me, &mr, out, &V);
-        p.partials = workspace(reduce_workspace_bytes(p), st);
-        check_hip(launch_reduce(p, V, st, &used), "reduction launch");
-        note_kernel(used);
-        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
-    } catch (...) {
-        (void)hipStreamSynchronize(st);
-        (void)hipFree(tmp);
-        throw;
-    }
-    check_hip(hipFree(tmp), "hipFree");
-    return true;
-}
-}  // namespace
-
 xg_status xg_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, xg_stream_t stream) {
-    return guard([&] {
-        hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-        if (r && (r->op == XG_RED_VARIANCE || r->op == XG_RED_STDDEV)) {
-            reduce_moment(e, r, out, st);
-            return;
-        }
-        if (r && r->op == XG_RED_MINMAX) {
-            reduce_minmax(e, r, out, st);
-            return;
-        }
-        int V = 4;
-        RedPlan p{};
-        try {
-            p = plan_reduce_best(e, r, out, &V);
-        } catch (const Error& err) {
-            if (err.code != XG_UNSUPPORTED || !reduce_in_pieces(e, r, out, st)) throw;
-            return;
-        }
-        if (p.n_out == 0) {
-            note_kernel("empty");
-            return;
-        }
-        p.partials = workspace(reduce_workspace_bytes(p), st);
-        const char* used = nullptr;
-        check_hip(launch_reduce(p, V, st, &used), "reduction launch");
-        note_kernel(used);
-    });
+    return guard([&] { reduce_call(e, r, out, reinterpret_cast<hipStream_t>(stream), stream_workspace, nullptr); });
 }
 
 xg_status xg_scan_info(const xg_expr* e, const xg_scan_desc* s, int32_t* dtype, int32_t* ndim, int64_t* shape) {
@@ -613,6 +731,8 @@ xg_status xg_plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_ten
             p->kind = 1;
             if (r && (r->op == XG_RED_VARIANCE || r->op == XG_RED_STDDEV || r->op == XG_RED_MINMAX))
                 raise(XG_UNSUPPORTED, "variance, stddev and minmax run several passes; use xg_reduce");
+            if (r && r->flags)
+                raise(XG_UNSUPPORTED, "immediate reductions evaluate their expression first; use xg_reduce");
             p->red = plan_reduce_best(e, r, out, &p->V);
             p->ws = reduce_workspace_bytes(p->red);
         } catch (...) {
@@ -750,22 +870,66 @@ xg_status xg_debug_assign_pieces(const xg_expr* e, const xg_tensor* out, int64_t
                                  int64_t* max_units) {
     return guard([&] {
         if (!n_pieces || !all_compact || !max_units) raise(XG_INVALID_ARGUMENT, "null output pointer");
-        ew_set_force_dynamic(1);
+        // this thread only, restored afterwards: other threads keep planning as they were
+        const int prev = ew_set_thread_force_dynamic(1);
         std::vector<Piece> pieces;
         bool ok = false;
         try {
             ok = plan_in_pieces(e, out, &pieces);
         } catch (...) {
-            ew_set_force_dynamic(0);
+            ew_set_thread_force_dynamic(prev);
             throw;
         }
-        ew_set_force_dynamic(0);
+        ew_set_thread_force_dynamic(prev);
         *n_pieces = ok ? int64_t(pieces.size()) : 0;
         *all_compact = 1;
         *max_units = 0;
         for (const Piece& pc : pieces) {
             if (!pc.first.compact) *all_compact = 0;
             *max_units = std::max<int64_t>(*max_units, int64_t(pc.first.units));
+        }
+    });
+}
+
+// Testing aid (host only, no device access): how a reduction whose reduced
+// extent is too long for one plan is cut.  *n_pieces = the number of pieces
+// (0: one plan fits, not cut), *axis = the axis cut into chunks, *piece_len =
+// its chunk length, *all_fit = 1 if every piece and the merge plan without
+// needing 64-bit dimensions.  Planned with the generic kernels (no run-time
+// compiles) and an assumed 256 compute units.
+xg_status xg_debug_reduce_pieces(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, int64_t* n_pieces,
+                                 int32_t* axis, int64_t* piece_len, int32_t* all_fit) {
+    return guard([&] {
+        if (!n_pieces || !axis || !piece_len || !all_fit) raise(XG_INVALID_ARGUMENT, "null output pointer");
+        *n_pieces = 0;
+        *axis = -1;
+        *piece_len = 0;
+        *all_fit = 1;
+        constexpr int kCus = 256;
+        try {
+            plan_reduce(e, r, out, 4, kCus, nullptr, nullptr);
+            return;
+        } catch (const Error& err) {
+            if (err.code != XG_UNSUPPORTED) throw;
+        }
+        const RedFinish fin = reduce_finish(e, r);
+        ReducePieces P(e, r, out, fin.vdt);
+        if (!P.ok) {
+            *all_fit = 0;
+            return;
+        }
+        *n_pieces = P.k_pieces;
+        *axis = P.d;
+        *piece_len = P.len_max;
+        char* tmp = reinterpret_cast<char*>(kSizingBase);
+        for (int64_t q = 0; q < P.k_pieces; ++q) {
+            P.select(q, tmp);
+            try {
+                plan_reduce(&P.pe, &P.pr, &P.o, 4, kCus, nullptr, nullptr);
+            } catch (const Error& err) {
+                if (err.code != XG_UNSUPPORTED) throw;
+                *all_fit = 0;
+            }
         }
     });
 }

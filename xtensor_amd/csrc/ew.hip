@@ -140,20 +140,29 @@ std::vector<SpecEntry>& spec_table() {
     return t;
 }
 
-int g_force_dynamic = 0;
+int g_force_dynamic_all = 0;              // every thread (testing aid)
+thread_local int t_force_dynamic = 0;     // the calling thread only
 int g_units_per_thread = 0;  // 0: choose by size; 1 or 4: forced (measurement aid)
+
+bool force_dynamic() { return g_force_dynamic_all || t_force_dynamic; }
 
 }  // namespace
 
-void ew_set_force_dynamic(int on) { g_force_dynamic = on; }
+void ew_set_force_dynamic(int on) { g_force_dynamic_all = on; }
+
+int ew_set_thread_force_dynamic(int on) {
+    const int prev = t_force_dynamic;
+    t_force_dynamic = on ? 1 : 0;
+    return prev;
+}
 
 bool ew_jit_ok(const EwPlan& p, int V) {
-    return !g_force_dynamic && p.compact && jit_wanted(p.units * uint64_t(V)) && jit_ew(p, V) != nullptr;
+    return !force_dynamic() && p.compact && jit_wanted(p.units * uint64_t(V)) && jit_ew(p, V) != nullptr;
 }
 void ew_set_units_per_thread(int upt) { g_units_per_thread = (upt == 1 || upt == 4) ? upt : 0; }
 
 const char* ew_find_spec(const EwPlan& p, int V) {
-    if (g_force_dynamic || !p.compact) return nullptr;
+    if (force_dynamic() || !p.compact) return nullptr;
     SpecKey k = spec_key(p);
     for (auto& s : spec_table())
         if (s.V == V && spec_equal(s.key, k)) return s.name;
@@ -170,7 +179,7 @@ int ew_grid(uint64_t units, int cus) {
 }
 
 int ew_spec_index(const EwPlan& p, int V) {
-    if (g_force_dynamic || !p.compact) return -1;
+    if (force_dynamic() || !p.compact) return -1;
     SpecKey k = spec_key(p);
     auto& t = spec_table();
     for (size_t i = 0; i < t.size(); ++i)
@@ -214,7 +223,7 @@ hipError_t launch_ew_multi(int spec_index, int rank, const CompactPlan* d_plans,
 hipError_t launch_ew(const EwPlan& p, int V, int cus, hipStream_t stream, const char** used) {
     if (p.units == 0) return hipSuccess;
     dim3 grid(ew_grid(p.units, cus)), block(256);
-    if (!g_force_dynamic && p.compact) {
+    if (!force_dynamic() && p.compact) {
         SpecKey k = spec_key(p);
         for (auto& s : spec_table()) {
             if (s.V == V && spec_equal(s.key, k)) {

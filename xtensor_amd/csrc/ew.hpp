@@ -18,6 +18,9 @@ const char* ew_find_spec(const EwPlan& p, int V);
 
 // Route every launch through the dynamic interpreter (testing aid).
 void ew_set_force_dynamic(int on);
+// The same for plans made and launched on the calling thread only; returns
+// the previous setting of that thread.
+int ew_set_thread_force_dynamic(int on);
 // True when a plan without an ahead-of-time program runs a run-time compiled
 // one (compiling it on first use).
 bool ew_jit_ok(const EwPlan& p, int V);

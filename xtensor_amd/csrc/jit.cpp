@@ -203,6 +203,8 @@ bool compile(const Env& e, const std::string& src, const std::string& expr, std:
     return ok;
 }
 
+thread_local int t_loaded_only = 0;
+
 // compiled (or cached) kernel `expr` of source `src` on the current device
 const JitKernel* get(const std::string& src, const std::string& expr) {
     const Env& e = env();
@@ -216,6 +218,10 @@ const JitKernel* get(const std::string& src, const std::string& expr) {
     if (en.k.fn) {
         ++g_stats.memory_hits;
         return &en.k;
+    }
+    if (t_loaded_only) {  // not loaded yet: leave the entry untouched for later callers
+        if (!en.mod) g_entries.erase(key);
+        return nullptr;
     }
     const uint64_t h = fnv1a(expr + "\n" + src, e.salt);
     const std::string path = e.cache_dir.empty() ? "" : (fs::path(e.cache_dir) / (hex(h) + ".co")).string();
@@ -264,6 +270,12 @@ int jit_get_mode() {
         if (n && *n) g_min_elements = std::strtoull(n, nullptr, 10);
     }
     return g_mode;
+}
+
+int jit_set_thread_loaded_only(int on) {
+    const int prev = t_loaded_only;
+    t_loaded_only = on ? 1 : 0;
+    return prev;
 }
 
 bool jit_wanted(uint64_t elements) {

@@ -34,6 +34,11 @@ void jit_set_mode(int mode);
 int jit_get_mode();
 // whether a launch over `elements` elements should use a compiled kernel
 bool jit_wanted(uint64_t elements);
+// On the calling thread only: 1 = hand out run-time compiled kernels only
+// when this process has already loaded them (no compile, no disk read, no
+// module load, so nothing is allocated on the device); 0 = normal.  Returns
+// the previous setting.
+int jit_set_thread_loaded_only(int on);
 
 // compiled kernel for a compact elementwise plan at vector width V, or nullptr
 const JitKernel* jit_ew(const EwPlan& p, int V);

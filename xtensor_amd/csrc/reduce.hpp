@@ -95,6 +95,19 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
 // Result dtype / shape only.
 void reduce_result(const xg_expr* e, const xg_reduce_desc* r, int* dtype, Shape* shape);
 
+// How the folded value of a reduction becomes its result: value type, the
+// finishing step, the type and divisor of a mean, and the result type.  Host
+// only and independent of the iteration space, so it also serves reductions
+// that are evaluated in pieces and merged.
+struct RedFinish {
+    int32_t vdt;
+    int32_t rdt;
+    int32_t fin;       // Finalize
+    int32_t mdt;
+    uint64_t div_bits;
+};
+RedFinish reduce_finish(const xg_expr* e, const xg_reduce_desc* r);
+
 #if !defined(__HIPCC_RTC__)  // host API (absent from run-time compiled sources)
 hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** used);
 #endif

@@ -262,6 +262,17 @@ std::vector<RDim> merge_dims(const std::vector<RDim>& d, int nleaves) {
     return m;
 }
 
+// static_cast<T-or-double>((size - ddof) / outputs), in size_t arithmetic
+// (detail::mean / mean_division, core/xmath.hpp:1827-1850), then converted
+// to the type of the division: the divisor in register form of t.mdt
+uint64_t mean_divisor_bits(const RedTypes& t, int64_t total, int64_t n_out, int64_t ddof) {
+    if (ddof < 0 || ddof > total) raise(XG_INVALID_ARGUMENT, "ddof out of range");
+    xg_scalar q{};
+    q.dtype = XG_UINT64;
+    q.v.u = n_out != 0 ? uint64_t(total - ddof) / uint64_t(n_out) : 0;
+    return scalar_bits(scalar_of_bits(scalar_bits(q, t.div_dt), t.div_dt), t.mdt);
+}
+
 uint32_t pow2_ceil(uint64_t x) {
     uint32_t p = 1;
     while (p < x && p < (1u << 30)) p <<= 1;
@@ -292,6 +303,28 @@ void reduce_result(const xg_expr* e, const xg_reduce_desc* r, int* dtype, Shape*
     }
     *dtype = t.rdt;
     *shape = rs;
+}
+
+RedFinish reduce_finish(const xg_expr* e, const xg_reduce_desc* r) {
+    if (!r) raise(XG_INVALID_ARGUMENT, "null reduce descriptor");
+    if (r->op == XG_RED_MINMAX) raise(XG_INVALID_ARGUMENT, "minmax is planned as two folds");
+    ExprInfo info = analyse(e);
+    const int nd = int(info.shape.size());
+    const std::vector<int> ax = norm_axes(r, nd);
+    const RedTypes t = red_types(r->op, r->acc_dtype, info.dtype);
+    int64_t n_out = 1;
+    for (int k = 0; k < nd; ++k)
+        if (std::find(ax.begin(), ax.end(), k) == ax.end()) n_out *= info.shape[size_t(k)];
+    RedFinish f{};
+    f.vdt = t.vdt;
+    f.rdt = t.rdt;
+    f.fin = t.fin;
+    f.mdt = t.mdt;
+    if (t.fin == FZ_MEAN || t.fin == FZ_MEAN_SQRT) {
+        const bool moment = r->op == XG_RED_VARIANCE || r->op == XG_RED_STDDEV;
+        f.div_bits = mean_divisor_bits(t, numel(info.shape), n_out, moment ? r->ddof : 0);
+    }
+    return f;
 }
 
 RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, int V, int cus,
@@ -346,16 +379,8 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
     int64_t n_out = numel(rs);
     int64_t total = numel(in);
     if (t.fin == FZ_MEAN || t.fin == FZ_MEAN_SQRT) {
-        // static_cast<T-or-double>((size - ddof) / outputs), in size_t arithmetic
-        // (detail::mean / mean_division, core/xmath.hpp:1827-1850), then
-        // converted to the type of the division
-        const int64_t ddof = moment ? r->ddof : 0;
-        if (ddof < 0 || ddof > total) raise(XG_INVALID_ARGUMENT, "ddof out of range");
-        xg_scalar q{};
-        q.dtype = XG_UINT64;
-        q.v.u = n_out != 0 ? uint64_t(total - ddof) / uint64_t(n_out) : 0;
         p.mdt = t.mdt;
-        p.div_bits = scalar_bits(scalar_of_bits(scalar_bits(q, t.div_dt), t.div_dt), t.mdt);
+        p.div_bits = mean_divisor_bits(t, total, n_out, moment ? r->ddof : 0);
     }
     if (n_out == 0) {
         p.n_out = 0;

@@ -19,6 +19,29 @@ void check_hip(hipError_t e, const char* what);
 // per (device, stream) scratch buffer of at least `bytes`
 void* workspace(size_t bytes, hipStream_t stream);
 
+// Device memory one call needs beyond its inputs and outputs, handed out by
+// a bump allocator over a single block.  A call runs twice: a sizing pass
+// (`measure`) plans every launch against placeholder addresses, launches
+// nothing and records the bytes used; the real pass then takes the same
+// regions, in the same order, from a block of that size.
+struct Arena {
+    bool measure = true;
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    // `bytes` (256-byte aligned) from the block; nullptr for 0 bytes
+    char* take(size_t bytes);
+};
+
+// Where the block of a call comes from: `get(bytes)` returns at least
+// `bytes` of device memory usable in stream order on the call's stream.
+using BlockSource = char* (*)(size_t bytes, hipStream_t stream, void* ctx);
+
+// out = reduce(e) (every reduction kind of xg_reduce), with its temporaries
+// and split partials in one block from `src`.
+void reduce_call(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t stream,
+                 BlockSource src, void* ctx);
+
 // Plan out = e with the compiled-kernel vector width, falling back to the
 // generic interpreter's width when no compiled program matches.
 EwPlan plan_assign_best(const xg_expr* e, const xg_tensor* out, int* V);
