@@ -267,8 +267,8 @@ const char* xg_version(void);
 const char* xg_last_error_message(void);
 
 /* Initialise the library on the current HIP device (queries the device,
- * allocates the per-device workspace pool).  Optional: every entry point
- * initialises lazily. */
+ * resolves the device address of xg_reduce_noalloc's scratch array).
+ * Optional: every entry point initialises lazily. */
 xg_status xg_init(void);
 
 /* Shape and value type of an expression: broadcast shape of all leaves
@@ -296,6 +296,23 @@ xg_status xg_reduce_info(const xg_expr* e, const xg_reduce_desc* r, int32_t* dty
  * amin accept a reduced dimension past 2^31 elements: it is reduced in
  * pieces through a temporary of the result's size per piece. */
 xg_status xg_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, xg_stream_t stream);
+
+/* out = reduce(e, axes) exactly as xg_reduce, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in xtensor.  The call's
+ * temporaries (split partials, a variance's kept-dims mean, an immediate
+ * reduction's evaluated expression, the pieces of a reduction past 32-bit
+ * indexing) live in one fixed array per device of
+ * xg_reduce_noalloc_max_bytes() bytes (16 MiB), held until the process ends;
+ * a reduction that needs more returns XG_OUT_OF_MEMORY before enqueueing
+ * anything.  It calls no hipMalloc / hipFree and loads no module: a run-time
+ * compiled kernel is used only if this process has already loaded it (else
+ * the ahead-of-time or generic kernels run).  Every call on one device
+ * shares that array: calls that overlap in time on one device (two streams,
+ * or two threads) race and must be serialised by the caller. */
+xg_status xg_reduce_noalloc(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, xg_stream_t stream);
+/* Size in bytes of the per-device array xg_reduce_noalloc uses. */
+size_t xg_reduce_noalloc_max_bytes(void);
 
 /* Result dtype and shape of a scan of e. */
 xg_status xg_scan_info(const xg_expr* e, const xg_scan_desc* s, int32_t* dtype, int32_t* ndim, int64_t* shape);

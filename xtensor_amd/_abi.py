@@ -149,6 +149,8 @@ EXPORTS = {
     "xg_assign": (C.c_int, [_P(Expr), _P(Tensor), C.c_void_p]),
     "xg_reduce_info": (C.c_int, [_P(Expr), _P(ReduceDesc), _P(C.c_int32), _P(C.c_int32), _P(C.c_int64)]),
     "xg_reduce": (C.c_int, [_P(Expr), _P(ReduceDesc), _P(Tensor), C.c_void_p]),
+    "xg_reduce_noalloc": (C.c_int, [_P(Expr), _P(ReduceDesc), _P(Tensor), C.c_void_p]),
+    "xg_reduce_noalloc_max_bytes": (C.c_size_t, []),
     "xg_scan_info": (C.c_int, [_P(Expr), _P(ScanDesc), _P(C.c_int32), _P(C.c_int32), _P(C.c_int64)]),
     "xg_scan": (C.c_int, [_P(Expr), _P(ScanDesc), _P(Tensor), C.c_void_p]),
     "xg_assign_indexed": (C.c_int, [_P(Expr), _P(Expr), _P(Tensor), C.c_void_p]),

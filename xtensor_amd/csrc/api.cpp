@@ -577,7 +577,10 @@ const char* xg_last_error_message(void) { return t_err.c_str(); }
 const char* xg_last_kernel(void) { return t_last_kernel.c_str(); }
 
 xg_status xg_init(void) {
-    return guard([&] { device_cus(); });
+    return guard([&] {
+        device_cus();
+        scratch_setup();
+    });
 }
 
 xg_status xg_expr_info(const xg_expr* e, int32_t* dtype, int32_t* ndim, int64_t* shape) {

@@ -18,6 +18,9 @@ void note_kernel(const char* name);
 void check_hip(hipError_t e, const char* what);
 // per (device, stream) scratch buffer of at least `bytes`
 void* workspace(size_t bytes, hipStream_t stream);
+// device address of xg_reduce_noalloc's fixed per-device array, resolved
+// once per device (the current one)
+char* scratch_setup();
 
 // Device memory one call needs beyond its inputs and outputs, handed out by
 // a bump allocator over a single block.  A call runs twice: a sizing pass
