@@ -175,3 +175,159 @@ def test_scans_and_reductions_across_dtypes(dtype):
                 assert got.dtype == ref.dtype, (red.__name__, ax, got.dtype, ref.dtype)
                 assert np.array_equal(got, ref) or (got.dtype.kind == "f" and np.allclose(got, ref, rtol=1e-14, atol=0)), \
                     (red.__name__, ax)
+
+
+def _scan_ref(x, axis, kind):
+    """numpy's sequential scan, for floats in long double (x86 80-bit: its
+    own rounding stays far below the float64 bar), rounded to float64."""
+    f = np.cumsum if kind == "cumsum" else np.cumprod
+    if x.dtype.kind == "f":
+        return f(x.astype(np.longdouble), axis=axis).astype(np.float64)
+    return f(x, axis=axis, dtype=x.dtype)  # C++: an int scan stays int (wrapping)
+
+
+def _check_scan(got, ref, dtype, what, kind="cumsum", length=1):
+    """Sums: rel 2e-7 (float32, compensated) / 1e-14 (float64).  Products are
+    not compensated: a product of n factors carries up to about n rounding
+    errors of the type (n * 2^-24 or n * 2^-53 relative) in any order, the
+    sequential one included."""
+    if np.dtype(dtype).kind == "f":
+        err = np.max(np.abs(got.astype(np.float64) - ref) / np.maximum(np.abs(ref), 1.0)) if ref.size else 0.0
+        if kind == "cumprod":
+            bar = length * (2.0 ** -24 if dtype == "float32" else 2.0 ** -53)
+        else:
+            bar = 2e-7 if dtype == "float32" else 1e-14
+        assert err <= bar, (what, err, bar)
+    else:
+        assert got.dtype == ref.dtype and np.array_equal(got, ref), what
+
+
+def _tiled_input(rng, shape, dtype, kind):
+    if kind == "cumprod":  # products that stay in range: +-1 for integers, near 1 for floats
+        if np.dtype(dtype).kind == "f":
+            return rng.uniform(0.999, 1.001, shape).astype(dtype)
+        if np.dtype(dtype).kind == "u":  # products of 3s wrap modulo 2^32, as in C++
+            return rng.choice(np.array([1, 1, 1, 3], dtype=dtype), shape)
+        return rng.choice(np.array([-1, 1, 1, 1], dtype=dtype), shape)
+    if np.dtype(dtype).kind == "f":
+        return rng.normal(1.0, 1.0, shape).astype(dtype)
+    hi = 1000 if np.dtype(dtype).kind == "i" else 2000
+    lo = -1000 if np.dtype(dtype).kind == "i" else 0
+    return rng.integers(lo, hi, shape).astype(dtype)
+
+
+# (shape, tiled): the tiled kernels need rows of whole 16-byte vectors
+TILED_SHAPES = [((8192, 4096), True), ((1000, 300), True), ((77, 8), True), ((3, 1000, 132), True),
+                ((130, 2, 260), True), ((77, 5), False), ((130, 2, 257), False)]
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64", "int32", "int64", "uint32"])
+@pytest.mark.parametrize("kind", ["cumsum", "cumprod"])
+def test_tiled_column_scans(dtype, kind):
+    """Scans along a strided axis whose other axis is contiguous (cumsum(X, 0)
+    of a row-major X, and the middle axis of 3-d arrays) run the tiled
+    kernels: chunk totals, carries across chunks, scan from the carry.  Shapes
+    cover whole and partial column tiles (256 columns of 4-byte types, 128 of
+    8-byte types), partial chunks and rows shorter than one vector.  Integers
+    exact; floats against a float64 scan (rel 2e-7 float32, 1e-14 float64)."""
+    rng = np.random.default_rng(31)
+    for shape, tiled in TILED_SHAPES:
+        if kind == "cumprod" and shape[0] * (shape[-1] if len(shape) == 2 else shape[1]) > 4_000_000:
+            continue
+        axis = 0 if len(shape) == 2 else 1
+        x = _tiled_input(rng, shape, dtype, kind)
+        X = xt.from_numpy(x)
+        got = getattr(xt, kind)(X, axis)
+        sync()
+        assert (xt.last_kernel() == "scan_tiles") == tiled, (shape, xt.last_kernel())
+        _check_scan(got.to_numpy(), _scan_ref(x, axis, kind), dtype, (kind, shape), kind, shape[axis])
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64", "int32", "int64"])
+def test_tiled_transposed_scans(dtype):
+    """cumsum(transpose(X), 1): the input is contiguous across scan lines and
+    the output along them; the tiled kernel goes through an LDS transpose."""
+    rng = np.random.default_rng(32)
+    for shape, tiled in [((8192, 4096), True), ((1000, 300), True), ((36, 8), True), ((8, 2000), True),
+                         ((33, 7), False), ((5, 2000), False)]:
+        x = _tiled_input(rng, shape, dtype, "cumsum")
+        X = xt.from_numpy(x)
+        got = xt.cumsum(xt.transpose(X), 1)
+        sync()
+        assert (xt.last_kernel() == "scan_tiles_trans") == tiled, (shape, xt.last_kernel())
+        _check_scan(got.to_numpy(), _scan_ref(x.T, 1, "cumsum"), dtype, shape)
+
+
+def test_tiled_scan_store_conversion_and_plans():
+    """A tiled scan into an output of another type of the same size (float32
+    sums stored as int32, static_cast), and the same scan as a prepared plan
+    replayed on two inputs, exact against the oracle."""
+    rng = np.random.default_rng(33)
+    x = rng.integers(-50, 50, (600, 300)).astype(np.float32)
+    X = Dual.of(x)
+    out = xt.empty((600, 300), "int32")
+    xt.plan(out, xt.Scan("cumsum", X.dev, 0)).run()
+    sync()
+    assert xt.last_kernel() == "scan_tiles"
+    ref = oracle_for(X).scan(X.dev, 0)
+    assert np.array_equal(out.to_numpy(), ref.astype(np.int32))
+    y = rng.integers(-50, 50, (600, 300)).astype(np.float32)
+    Y = xt.from_numpy(y)
+    o2 = xt.empty((600, 300), "float32")
+    p = xt.plan(o2, xt.Scan("cumsum", Y, 0))
+    for _ in range(2):
+        p.run()
+        sync()
+        assert np.array_equal(o2.to_numpy(), np.cumsum(y.astype(np.float64), axis=0).astype(np.float32))
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64", "int32", "int64"])
+def test_lookback_long_rows(dtype):
+    """Few rows much longer than one chunk (4096 elements of 4-byte types,
+    2048 of 8-byte types) run the single-pass look-back scan: the flattened
+    cumsum of a large array, and cumsum along axis 1 of a few long rows.
+    Lengths around the chunk edges and one past 2^25.  Integers exact; floats
+    against a float64 scan (rel 2e-7 for float32, 1e-14 for float64); the
+    float64 and int64 cases use the flag + value publication, the others the
+    packed word."""
+    rng = np.random.default_rng(41)
+    for n in [4097, 8192, 8193, 300_001, (1 << 25) + 7]:
+        x = _tiled_input(rng, (n,), dtype, "cumsum")
+        if np.dtype(dtype).kind != "f":
+            x = (x % 7 - 3).astype(dtype)
+        X = xt.from_numpy(x)
+        got = xt.cumsum(X)
+        sync()
+        assert xt.last_kernel() == "scan_lookback", (n, xt.last_kernel())
+        _check_scan(got.to_numpy(), _scan_ref(x, 0, "cumsum"), dtype, n)
+    x = _tiled_input(rng, (3, 1_000_004), dtype, "cumsum")  # rows of whole 16-byte vectors
+    X = xt.from_numpy(x)
+    got = xt.cumsum(X, 1)
+    sync()
+    assert xt.last_kernel() == "scan_lookback"
+    _check_scan(got.to_numpy(), _scan_ref(x, 1, "cumsum"), dtype, "3 rows")
+
+
+def test_lookback_cumprod_repeat_and_expression():
+    """cumprod of +-1 integers and of values near 1 through the look-back
+    kernel; the same scan twice in a row (the chunk flags are cleared before
+    every launch); the flattened scan of an expression (evaluated into a
+    temporary that shares the workspace block with the chunk flags)."""
+    rng = np.random.default_rng(42)
+    p = rng.choice(np.array([-1, 1, 1, 1], np.int32), 200_000)
+    P = xt.from_numpy(p)
+    for _ in range(2):
+        got = xt.cumprod(P).to_numpy()
+        assert xt.last_kernel() == "scan_lookback"
+        assert np.array_equal(got, np.cumprod(p))
+    f = rng.uniform(0.9999, 1.0001, 500_000).astype(np.float64)
+    got = xt.cumprod(xt.from_numpy(f)).to_numpy()
+    ref = _scan_ref(f, 0, "cumprod")
+    assert np.max(np.abs(got - ref) / np.abs(ref)) <= 1e-13
+    a = rng.normal(0, 1, (300, 1000)).astype(np.float32)
+    A_ = Dual.of(a)
+    e = A_.dev * 2.0 + 1.0  # float64 values (double literals)
+    got = xt.cumsum(e).to_numpy()
+    assert xt.last_kernel() == "scan_lookback"
+    ref = _scan_ref((a.astype(np.float64) * 2.0 + 1.0).ravel(), 0, "cumsum")
+    assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)) <= 1e-14

@@ -13,10 +13,6 @@
 
 namespace xg {
 
-struct Store {
-    uint64_t a, b;
-};
-
 // ---------------------------------------------------------------- accumulators
 template <int AK, int OP> struct Acc;
 

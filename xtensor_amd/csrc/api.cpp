@@ -638,41 +638,44 @@ xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out,
         scan_result(e, s, &dt, &rs);
         const int V = 4;
         if (s->axis != XG_AXIS_NONE) {
-            ScanPlan p = plan_scan(e, s, out, V);
+            ScanPlan p = plan_scan(e, s, out, V, device_cus());
             const char* used = nullptr;
-            check_hip(launch_scan(p, V, st, &used), "scan launch");
+            check_hip(launch_scan(p, V, workspace(scan_workspace_bytes(p), st), st, &used), "scan launch");
             note_kernel(used);
             return;
         }
-        // flattened scan: evaluate e row-major into a contiguous temporary of
-        // the result type, then scan it as one row
+        // flattened scan (accumulator_impl without an axis): a lone leaf that
+        // is contiguous in row-major order is scanned where it lies; any other
+        // expression is evaluated row-major into a contiguous temporary of the
+        // result type first, as xtensor copies it before scanning
         if (out->ndim != 1 || out->shape[0] != rs[0]) raise(XG_BROADCAST_ERROR, "flattened scan output must be 1-d");
         ExprInfo info = analyse(e);
         int64_t n = numel(info.shape);
         if (n == 0) return;
-        xg_tensor tmp{};
-        tmp.dtype = dt;
-        tmp.ndim = int32_t(info.shape.size());
-        int64_t stv = 1;
-        for (int k = tmp.ndim - 1; k >= 0; --k) {
-            tmp.shape[k] = info.shape[size_t(k)];
-            tmp.strides[k] = stv;
-            stv *= info.shape[size_t(k)];
+        const xg_tensor* lone = (e->n_nodes == 1 && e->nodes[0].op == XG_OP_LEAF && e->leaves[e->nodes[0].index].dtype == dt)
+                                    ? &e->leaves[e->nodes[0].index]
+                                    : nullptr;
+        if (lone) {
+            int64_t expect = 1;
+            for (int k = lone->ndim - 1; k >= 0 && lone; --k) {
+                if (lone->shape[k] != 1 && lone->strides[k] != expect) lone = nullptr;
+                else expect *= lone->shape[k];
+            }
         }
-        tmp.data = workspace(size_t(n) * dtype_size(dt), st);
-        check_hip(hipSuccess, "");
-        {
-            int Vw = 4;
-            EwPlan ep = plan_assign_best(e, &tmp, &Vw);
-            check_hip(launch_ew(ep, Vw, device_cus(), st, nullptr), "scan copy launch");
-        }
-        // 1-d view of the temporary as the only leaf
         xg_tensor flat{};
-        flat.data = tmp.data;
-        flat.dtype = dt;
         flat.ndim = 1;
         flat.shape[0] = n;
         flat.strides[0] = 1;
+        size_t tmp_bytes = 0;
+        if (lone) {
+            flat.data = lone->data;
+            flat.dtype = lone->dtype;
+            flat.offset = lone->offset;
+        } else {
+            flat.dtype = dt;
+            tmp_bytes = (size_t(n) * size_t(dtype_size(dt)) + 255) & ~size_t(255);
+            flat.data = reinterpret_cast<void*>(kSizingBase);  // sizing pass: an aligned placeholder
+        }
         xg_node node{};
         node.op = XG_OP_LEAF;
         xg_expr fe{};
@@ -682,9 +685,30 @@ xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out,
         fe.leaves = &flat;
         xg_scan_desc s1 = *s;
         s1.axis = 0;
-        ScanPlan p = plan_scan(&fe, &s1, out, V);
+        const int cus = device_cus();
+        ScanPlan p = plan_scan(&fe, &s1, out, V, cus);
+        const size_t wsb = scan_workspace_bytes(p);
+        char* block = tmp_bytes + wsb ? static_cast<char*>(workspace(tmp_bytes + wsb, st)) : nullptr;
+        if (!lone) {
+            xg_tensor tmp{};
+            tmp.data = block;
+            tmp.dtype = dt;
+            tmp.ndim = int32_t(info.shape.size());
+            int64_t stv = 1;
+            for (int k = tmp.ndim - 1; k >= 0; --k) {
+                tmp.shape[k] = info.shape[size_t(k)];
+                tmp.strides[k] = stv;
+                stv *= info.shape[size_t(k)];
+            }
+            int Vw = 4;
+            EwPlan ep = plan_assign_best(e, &tmp, &Vw);
+            check_hip(launch_ew(ep, Vw, cus, st, nullptr), "scan copy launch");
+            flat.data = block;
+            p = plan_scan(&fe, &s1, out, V, cus);
+            if (scan_workspace_bytes(p) > wsb) raise(XG_INVALID_ARGUMENT, "internal: scan workspace grew after sizing");
+        }
         const char* used = nullptr;
-        check_hip(launch_scan(p, V, st, &used), "scan launch");
+        check_hip(launch_scan(p, V, wsb ? block + tmp_bytes : nullptr, st, &used), "scan launch");
         note_kernel(used);
     });
 }
@@ -754,7 +778,8 @@ xg_status xg_plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor*
         try {
             p->kind = 2;
             p->V = 4;
-            p->scan = plan_scan(e, s, out, p->V);
+            p->scan = plan_scan(e, s, out, p->V, device_cus());
+            p->ws = scan_workspace_bytes(p->scan);
         } catch (...) {
             delete p;
             throw;
@@ -779,7 +804,8 @@ xg_status xg_plan_execute(xg_plan_t p, void* ws, xg_stream_t stream) {
                 check_hip(launch_reduce(p->red, p->V, st, &used), "reduction launch");
             }
         } else {
-            check_hip(launch_scan(p->scan, p->V, st, &used), "scan launch");
+            void* w = p->ws ? (ws ? ws : workspace(p->ws, st)) : nullptr;
+            check_hip(launch_scan(p->scan, p->V, w, st, &used), "scan launch");
         }
         note_kernel(used);
     });
@@ -810,7 +836,8 @@ xg_status xg_plan_execute_many(const xg_plan_t* plans, int64_t n, xg_stream_t st
                 p->red.partials = p->ws ? workspace(p->ws, st) : nullptr;
                 check_hip(launch_reduce(p->red, p->V, st, &used), "reduction launch");
             } else if (p->kind == 2) {
-                check_hip(launch_scan(p->scan, p->V, st, &used), "scan launch");
+                check_hip(launch_scan(p->scan, p->V, p->ws ? workspace(p->ws, st) : nullptr, st, &used),
+                          "scan launch");
             }
         }
         note_kernel(ew.size() > 1 ? "multi" : "");

@@ -12,33 +12,56 @@
 
 namespace xg {
 
-// The direct rows kernel applies: rank <= 2 rows layout with 32-bit offsets,
-// a program that is one load of a leaf of the accumulation input type, the
-// leaf and the output contiguous along the axis, rows starting 16-byte
-// aligned, output elements of the same size, not bf16.
 static const int kScanInDtype[AK_NUM] = {XG_FLOAT32, XG_FLOAT64, XG_FLOAT32, XG_FLOAT64,
                                          XG_INT32,   XG_UINT32,  XG_INT64,   XG_UINT64};
 
-static bool scan_direct_ok(const ScanPlan& p) {
-    const int* in_dt = kScanInDtype;
-    if (p.layout != SL_ROWS || p.index64 || p.rank < 1 || p.rank > 2 || p.acc < 0 || p.acc >= AK_NUM) return false;
-    if (p.rank == 1 && p.rows != 1) return false;
-    if (p.prog.nops != 1 || p.prog.ops[0].op != MO_LOAD || p.prog.nleaves != 1) return false;
-    const LeafView& l = p.leaves[0];
-    const int esz = dtype_size(l.dtype);
-    if (l.dtype != in_dt[p.acc] || lm_base(l.mode) != LM_CONTIG || l.stride[p.rank - 1] != 1) return false;
-    if (p.out_dtype == XG_BFLOAT16 || dtype_size(p.out_dtype) != esz || p.out_stride[p.rank - 1] != 1) return false;
-    if (reinterpret_cast<uintptr_t>(l.ptr) % 16 || reinterpret_cast<uintptr_t>(p.out_ptr) % 16) return false;
-    if (p.rank == 2 && ((l.stride[0] * esz) % 16 || (p.out_stride[0] * esz) % 16)) return false;
-    return true;
-}
-
-hipError_t launch_scan(const ScanPlan& p, int V, hipStream_t stream, const char** used) {
+hipError_t launch_scan(const ScanPlan& p, int V, void* ws, hipStream_t stream, const char** used) {
     if (p.rows == 0) return hipSuccess;
     if (V != 4) return hipErrorInvalidValue;
     ScanKernels k{};
     if (!scan_kernels_float(p.acc, p.op, &k) && !scan_kernels_int(p.acc, p.op, &k)) return hipErrorInvalidValue;
     dim3 block(256);
+    if (p.tiled != ST_NONE) {
+        const bool trans = p.tiled == ST_TRANS;
+        TFn scan = k.tiles[p.tile_native ? 1 : 0][trans ? 1 : 0];
+        if (!scan) return hipErrorInvalidValue;
+        TileScan t = p.tile;
+        t.part = static_cast<Store*>(ws);
+        if (t.nchunks > 1 && !ws) return hipErrorInvalidValue;
+        const uint32_t blocks = t.outer * t.nchunks * t.ntiles;
+        if (t.nchunks > 1) {
+            hipLaunchKernelGGL(k.tile_totals, dim3(blocks), block, 0, stream, t);
+            hipLaunchKernelGGL(k.tile_carry, dim3(t.outer * ((t.ncols + 63u) / 64u)), block, 0, stream, t);
+        }
+        hipLaunchKernelGGL(scan, dim3(blocks), block, 0, stream, t);
+        if (used) *used = trans ? "scan_tiles_trans" : "scan_tiles";
+        return hipGetLastError();
+    }
+    if (p.lookback) {
+        LbScan l = p.lb;
+        if (!ws) return hipErrorInvalidValue;
+        char* w = static_cast<char*>(ws);
+        const size_t n = l.total;
+        auto a256 = [](size_t b) { return (b + 255) & ~size_t(255); };
+        l.ctrl = reinterpret_cast<uint32_t*>(w);
+        w += 256;
+        if (p.lb_packed) {
+            l.words = reinterpret_cast<uint64_t*>(w);
+        } else {
+            l.flags = reinterpret_cast<uint32_t*>(w);
+            w += a256(n * 4);
+            l.aggregate = reinterpret_cast<Store*>(w);
+            w += a256(n * sizeof(Store));
+            l.prefix = reinterpret_cast<Store*>(w);
+        }
+        const bool native = p.out_dtype == kScanInDtype[p.acc] && p.rdt == kScanInDtype[p.acc];
+        const uint32_t init_blocks = uint32_t((n + 255) / 256);
+        if (p.lb_packed) hipLaunchKernelGGL(scan_lookback_init_kernel<true>, dim3(init_blocks), block, 0, stream, l);
+        else hipLaunchKernelGGL(scan_lookback_init_kernel<false>, dim3(init_blocks), block, 0, stream, l);
+        hipLaunchKernelGGL(k.lookback[native ? 1 : 0], dim3(l.total), block, 0, stream, l);
+        if (used) *used = "scan_lookback";
+        return hipGetLastError();
+    }
     if (k.direct[0][0] && scan_direct_ok(p)) {
         const bool native = p.out_dtype == kScanInDtype[p.acc] && p.rdt == kScanInDtype[p.acc];
         // rows longer than one wave's tile take a block of four waves each

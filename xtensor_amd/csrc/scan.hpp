@@ -15,6 +15,45 @@ enum ScanLayout : int32_t {
     SL_COLS = 1   // scan axis strided: a thread per V adjacent columns walks the axis
 };
 
+// Tiled scans (scan_tiled.hpp): the input viewed as [outer, len, inner]
+// with `inner` contiguous, the scan along `len`.
+enum ScanTiled : int32_t {
+    ST_NONE = 0,
+    ST_COLS = 1,   // output contiguous along inner as well (cumsum(X, 0))
+    ST_TRANS = 2   // output contiguous along len (cumsum(transpose(X), 1))
+};
+
+// Kernel argument of the tiled scans (pointers filled in at launch).
+struct TileScan {
+    const char* in;
+    char* out;
+    Store* part;          // [outer][nchunks][ncols] chunk totals, then carries
+    uint32_t outer, len, inner;
+    uint32_t nchunks, ntiles, chunk_rows, ncols;  // ncols = ntiles * 64 * V
+    int64_t in_s_outer, in_s_len;                 // elements; inner stride 1
+    int64_t out_s_outer, out_s_len, out_s_inner;  // elements
+    uint64_t identity;
+    int32_t out_dtype, rdt;
+    int32_t nt_in, nt_out;
+};
+
+// Kernel argument of the look-back scan of long contiguous rows
+// (scan_lookback.hpp); pointers into the workspace filled in at launch.
+struct LbScan {
+    const char* in;
+    char* out;
+    uint32_t* ctrl;       // [0] ticket counter, [1] look-back timeouts
+    uint64_t* words;      // packed value + flag per chunk (32-bit accumulators)
+    uint32_t* flags;      // flag per chunk (other accumulators) ...
+    Store* aggregate;     // ... with the chunk totals
+    Store* prefix;        // ... and the inclusive prefixes
+    uint32_t rows, len, cpr, total, tile;  // cpr = chunks per row, total = rows * cpr
+    int64_t in_row_stride, out_row_stride; // elements
+    uint64_t identity;
+    int32_t out_dtype, rdt;
+    int32_t nt_in, nt_out;
+};
+
 struct ScanPlan {
     int32_t layout;
     int32_t rank;       // SL_ROWS: [outer..., axis]; SL_COLS: [outer..., axis, inner]
@@ -36,16 +75,34 @@ struct ScanPlan {
     int64_t out_stride[kMaxRank];
     LeafView leaves[kMaxLeaves];
     Program prog;       // element value converted to the accumulation input type
+    int32_t tiled;      // ScanTiled: the tiled kernels apply (then `tile` holds their shape)
+    int32_t tile_native;// result and output of the input's type
+    TileScan tile;
+    int32_t lookback;   // 1: few long rows, the look-back kernel (then `lb` holds its shape)
+    int32_t lb_packed;  // its publications are packed 64-bit words
+    LbScan lb;
 };
+
+// The direct rows kernels apply: rank <= 2 rows layout with 32-bit offsets,
+// a program that is one load of a leaf of the accumulation input type, the
+// leaf and the output contiguous along the axis, rows starting 16-byte
+// aligned, output elements of the same size, not bf16.
+bool scan_direct_ok(const ScanPlan& p);
 
 // Result dtype / shape of a scan.
 void scan_result(const xg_expr* e, const xg_scan_desc* s, int* dtype, Shape* shape);
 
-// Plan out = scan(e) along an axis (not the flattened form).
-ScanPlan plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, int V);
+// Plan out = scan(e) along an axis (not the flattened form); `cus` = compute
+// units of the device (sizes the tiled scans' chunks).
+ScanPlan plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, int V, int cus);
+
+// Bytes of workspace launch_scan needs for the plan (the tiled scans' chunk
+// totals; 0 for the other kernels).
+size_t scan_workspace_bytes(const ScanPlan& p);
 
 #if !defined(__HIPCC_RTC__)
-hipError_t launch_scan(const ScanPlan& p, int V, hipStream_t stream, const char** used);
+// `ws`: scan_workspace_bytes(p) bytes of device memory, usable in stream order.
+hipError_t launch_scan(const ScanPlan& p, int V, void* ws, hipStream_t stream, const char** used);
 #endif
 
 }  // namespace xg

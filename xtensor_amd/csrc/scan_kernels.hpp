@@ -323,16 +323,33 @@ __global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p)
     }
 }
 
+}  // namespace xg
+
+#include "scan_lookback.hpp"
+
+namespace xg {
+
 // elements of one wave's tile in the direct rows scan (rows up to this
 // length take one wave each)
 template <class T> constexpr uint32_t scan_wave_tile() { return 64u * uint32_t(16 / sizeof(T)) * 4u; }
 
+}  // namespace xg
+
+#include "scan_tiled.hpp"
+
+namespace xg {
+
 using SFn = void (*)(const ScanPlan);
+using TFn = void (*)(const TileScan);
+using LFn = void (*)(const LbScan);
 
 struct ScanKernels {
     SFn rows32, rows64, cols32, cols64;
     SFn direct[2][2];   // [native][waves per row: 1, 4]
     uint32_t wave_tile;  // elements of one wave's tile in the direct kernels
+    TFn tile_totals, tile_carry;
+    TFn tiles[2][2];     // [native][transposed output]; the transposed one is native only
+    LFn lookback[2];     // [native]
 };
 
 template <int AK, int OP> ScanKernels scan_kernels() {
@@ -342,7 +359,12 @@ template <int AK, int OP> ScanKernels scan_kernels() {
                        scan_cols_kernel<4, kMaxRegs, DynSpec, true, AK, OP>,
                        {{scan_rows_direct_kernel<AK, OP, false, 1>, scan_rows_direct_kernel<AK, OP, false, 4>},
                         {scan_rows_direct_kernel<AK, OP, true, 1>, scan_rows_direct_kernel<AK, OP, true, 4>}},
-                       scan_wave_tile<typename ScanIn<AK>::T>()};
+                       scan_wave_tile<typename ScanIn<AK>::T>(),
+                       scan_tiles_totals_kernel<AK, OP>,
+                       scan_tiles_carry_kernel<AK, OP>,
+                       {{scan_tiles_kernel<AK, OP, false, false>, nullptr},
+                        {scan_tiles_kernel<AK, OP, true, false>, scan_tiles_kernel<AK, OP, true, true>}},
+                       {scan_lookback_kernel<AK, OP, false>, scan_lookback_kernel<AK, OP, true>}};
 }
 
 bool scan_kernels_float(int ak, int op, ScanKernels* k);

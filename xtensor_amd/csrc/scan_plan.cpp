@@ -63,7 +63,150 @@ void scan_result(const xg_expr* e, const xg_scan_desc* s, int* dtype, Shape* sha
     *shape = info.shape;
 }
 
-ScanPlan plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, int V) {
+namespace {
+
+// element type the direct and tiled kernels read, per accumulator kind
+int scan_in_dtype(int acc) {
+    static const int t[AK_NUM] = {XG_FLOAT32, XG_FLOAT64, XG_FLOAT32, XG_FLOAT64,
+                                  XG_INT32,   XG_UINT32,  XG_INT64,   XG_UINT64};
+    return acc >= 0 && acc < AK_NUM ? t[acc] : -1;
+}
+
+// The tiled scans (scan_tiled.hpp) apply when the program is one load of a
+// leaf of the accumulation input type, the leaf is contiguous along a dim
+// other than the scan axis with 16-byte aligned rows, and the output is
+// contiguous along that dim (ST_COLS) or along the scan axis (ST_TRANS);
+// outputs of another element size or bf16 keep the other kernels.
+void plan_tiled(ScanPlan* pp, int cus) {
+    ScanPlan& p = *pp;
+    p.tiled = ST_NONE;
+    const int in_dt = scan_in_dtype(p.acc);
+    if (in_dt < 0 || p.prog.nops != 1 || p.prog.ops[0].op != MO_LOAD || p.prog.nleaves != 1 || p.prog.leaf_gather[0])
+        return;
+    const LeafView& l = p.leaves[0];
+    const int esz = dtype_size(in_dt);
+    const int64_t V = 16 / esz;
+    if (l.dtype != in_dt || p.out_dtype == XG_BFLOAT16 || dtype_size(p.out_dtype) != esz) return;
+    if (reinterpret_cast<uintptr_t>(l.ptr) % 16 || reinterpret_cast<uintptr_t>(p.out_ptr) % 16) return;
+    const bool native = p.out_dtype == in_dt && p.rdt == in_dt;
+    TileScan t{};
+    int kind = ST_NONE;
+    const int R = p.rank;
+    if (p.layout == SL_COLS && (R == 2 || R == 3)) {
+        if (l.stride[R - 1] != 1 || p.out_stride[R - 1] != 1) return;
+        t.outer = R == 3 ? uint32_t(p.shape[0]) : 1;
+        t.len = uint32_t(p.shape[R - 2]);
+        t.inner = uint32_t(p.shape[R - 1]);
+        t.in_s_outer = R == 3 ? l.stride[0] : 0;
+        t.in_s_len = l.stride[R - 2];
+        t.out_s_outer = R == 3 ? p.out_stride[0] : 0;
+        t.out_s_len = p.out_stride[R - 2];
+        t.out_s_inner = 1;
+        if (t.out_s_outer % V || t.out_s_len % V) return;
+        kind = ST_COLS;
+    } else if (p.layout == SL_ROWS && (R == 2 || R == 3) && native) {
+        if (l.stride[R - 2] != 1 || l.stride[R - 1] == 1 || p.out_stride[R - 1] != 1) return;
+        t.outer = R == 3 ? uint32_t(p.shape[0]) : 1;
+        t.inner = uint32_t(p.shape[R - 2]);
+        t.len = uint32_t(p.shape[R - 1]);
+        t.in_s_outer = R == 3 ? l.stride[0] : 0;
+        t.in_s_len = l.stride[R - 1];
+        t.out_s_outer = R == 3 ? p.out_stride[0] : 0;
+        t.out_s_len = 1;
+        t.out_s_inner = p.out_stride[R - 2];
+        if (t.out_s_outer % V || t.out_s_inner % V) return;
+        kind = ST_TRANS;
+    } else {
+        return;
+    }
+    if (t.in_s_outer % V || t.in_s_len % V || t.in_s_len < 0 || t.in_s_outer < 0) return;
+    const uint32_t sub = kind == ST_TRANS ? 32u : 64u;  // rows of one sub-tile (scan_tiles_kernel)
+    const uint64_t cols = uint64_t(64 * V);
+    t.ntiles = uint32_t((t.inner + cols - 1) / cols);
+    t.ncols = uint32_t(t.ntiles * cols);
+    // about four workgroups per compute unit, chunks of whole sub-tiles
+    const uint64_t per_chunk = uint64_t(t.outer) * t.ntiles;
+    const uint64_t target = uint64_t(std::max(cus, 1)) * 4;
+    uint64_t want = (target + per_chunk - 1) / per_chunk;
+    const uint64_t max_chunks = std::min<uint64_t>(256, (uint64_t(t.len) + sub - 1) / sub);
+    want = std::max<uint64_t>(1, std::min(want, max_chunks));
+    const uint64_t rows = (uint64_t(t.len) + want - 1) / want;
+    t.chunk_rows = uint32_t((rows + sub - 1) / sub * sub);
+    t.nchunks = uint32_t((uint64_t(t.len) + t.chunk_rows - 1) / t.chunk_rows);
+    if (per_chunk * t.nchunks >= (uint64_t(1) << 31)) return;
+    t.in = l.ptr;
+    t.out = p.out_ptr;
+    t.identity = p.identity;
+    t.out_dtype = p.out_dtype;
+    t.rdt = p.rdt;
+    t.nt_in = lm_nt(l.mode) ? 1 : 0;
+    t.nt_out = lm_nt(p.out_mode) ? 1 : 0;
+    p.tile = t;
+    p.tile_native = native ? 1 : 0;
+    p.tiled = kind;
+}
+
+// Rows too few to fill the GPU with a workgroup each, and longer than one
+// chunk, run the look-back kernel: chunks of 4 waves x 4 segments x 64 lanes
+// x 16 bytes (4096 float32 elements).
+void plan_lookback(ScanPlan* pp, int cus) {
+    ScanPlan& p = *pp;
+    p.lookback = 0;
+    if (!scan_direct_ok(p)) return;
+    const int in_dt = scan_in_dtype(p.acc);
+    const uint32_t tile = 16u * 64u * uint32_t(16 / dtype_size(in_dt));
+    if (p.len <= tile || uint64_t(p.rows) * 2 >= uint64_t(std::max(cus, 1)) * 4) return;
+    const uint64_t cpr = (uint64_t(p.len) + tile - 1) / tile;
+    if (cpr * p.rows >= (uint64_t(1) << 31)) return;
+    LbScan& l = p.lb;
+    l = LbScan{};
+    l.in = p.leaves[0].ptr;
+    l.out = p.out_ptr;
+    l.rows = p.rows;
+    l.len = p.len;
+    l.cpr = uint32_t(cpr);
+    l.total = uint32_t(cpr * p.rows);
+    l.tile = tile;
+    l.in_row_stride = p.rank == 2 ? p.leaves[0].stride[0] : 0;
+    l.out_row_stride = p.rank == 2 ? p.out_stride[0] : 0;
+    l.identity = p.identity;
+    l.out_dtype = p.out_dtype;
+    l.rdt = p.rdt;
+    l.nt_in = lm_nt(p.leaves[0].mode) ? 1 : 0;
+    l.nt_out = lm_nt(p.out_mode) ? 1 : 0;
+    p.lb_packed = (p.acc == AK_F32C || p.acc == AK_F32 || p.acc == AK_I32 || p.acc == AK_U32) ? 1 : 0;
+    p.lookback = 1;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+}  // namespace
+
+bool scan_direct_ok(const ScanPlan& p) {
+    if (p.layout != SL_ROWS || p.index64 || p.rank < 1 || p.rank > 2 || p.acc < 0 || p.acc >= AK_NUM) return false;
+    if (p.rank == 1 && p.rows != 1) return false;
+    if (p.prog.nops != 1 || p.prog.ops[0].op != MO_LOAD || p.prog.nleaves != 1 || p.prog.leaf_gather[0]) return false;
+    const LeafView& l = p.leaves[0];
+    const int esz = dtype_size(l.dtype);
+    if (l.dtype != scan_in_dtype(p.acc) || lm_base(l.mode) != LM_CONTIG || l.stride[p.rank - 1] != 1) return false;
+    if (p.out_dtype == XG_BFLOAT16 || dtype_size(p.out_dtype) != esz || p.out_stride[p.rank - 1] != 1) return false;
+    if (reinterpret_cast<uintptr_t>(l.ptr) % 16 || reinterpret_cast<uintptr_t>(p.out_ptr) % 16) return false;
+    if (p.rank == 2 && ((l.stride[0] * esz) % 16 || (p.out_stride[0] * esz) % 16)) return false;
+    return true;
+}
+
+size_t scan_workspace_bytes(const ScanPlan& p) {
+    if (p.rows == 0) return 0;
+    if (p.lookback) {
+        const size_t n = p.lb.total;
+        if (p.lb_packed) return 256 + align256(n * 8);
+        return 256 + align256(n * 4) + 2 * align256(n * sizeof(Store));
+    }
+    if (p.tiled == ST_NONE || p.tile.nchunks <= 1) return 0;
+    return size_t(p.tile.outer) * p.tile.nchunks * p.tile.ncols * sizeof(Store);
+}
+
+ScanPlan plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, int V, int cus) {
     if (!out || out->ndim < 0 || out->ndim > XG_MAX_RANK || out->dtype < 0 || out->dtype >= XG_NUM_DTYPES)
         raise(XG_INVALID_ARGUMENT, "malformed output tensor");
     ExprInfo info = analyse(e);
@@ -185,6 +328,8 @@ ScanPlan plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out
         if (lv.mode != LM_BCAST && streams(lv.stride, p.shape, p.rank, esz)) lv.mode |= LM_NT;
     }
     p.index64 = max_off >= (int64_t(1) << 31) ? 1 : 0;
+    plan_tiled(&p, cus);
+    if (p.tiled == ST_NONE) plan_lookback(&p, cus);
     return p;
 }
 

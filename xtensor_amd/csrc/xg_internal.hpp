@@ -35,6 +35,12 @@ XG_HD constexpr bool is_64bit(int dt) { return dt == XG_INT64 || dt == XG_UINT64
 // type a leaf of storage type `dt` has as a value (bf16 is storage only)
 XG_HD constexpr int value_dtype(int dt) { return dt == XG_BFLOAT16 ? XG_FLOAT32 : dt; }
 
+// A partial result of a reduction or scan tree as it sits in memory: a
+// compensated pair (sum, compensation), or a plain value in `a`.
+struct Store {
+    uint64_t a, b;
+};
+
 // ---------------------------------------------------------------- micro-ops
 // Registers are 64-bit.  A value of a type of at most 32 bits lives in the
 // low 32 bits (small integer types sign- or zero-extended to 32 bits as a
