@@ -92,23 +92,127 @@ def time_plans(torch, plans, steps, warmup):
     return wall, kms
 
 
-def cpu_baseline(np, budget_s=10.0):
+def _cpu_info():
+    """CPU model and logical CPU count of this host (the GPU box's host cores)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return model, n
+
+
+def _timed(f, args, budget_s):
+    """Seconds per repetition of f(*args, reps), reps sized to about budget_s."""
+    t1 = f(*args, 1)
+    reps = max(1, int(budget_s / max(t1, 1e-4)))
+    return f(*args, reps) / reps, reps
+
+
+def cpu_baseline(np, budget_s=5.0, per_s=1.5):
+    """The CPU restatements of xtensor's path for every workload, timed on
+    this host (oracle/cpu_baseline.cpp; xtensor itself is not buildable here).
+    Headline `value`: the affine + clip workload (xtensor's scalar stepper
+    loop, one core) on a sample of the images.  `per_config`: every other
+    workload, GB/s of the same algorithmic bytes as the GPU figures, with
+    the threads each used (OpenMP over all cores where xtensor's TBB loop
+    would run, one core where xtensor's loop is single-threaded)."""
     import ctypes
     lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "_build", "libcpubase.so"))
-    f = lib.xb_affine_clip_f32
-    f.restype = ctypes.c_double
+    for name in ("xb_affine_clip_f32", "xb_bitmix_i32", "xb_mul_add_exp_f64", "xb_row_sum_f32", "xb_col_sum_f32",
+                 "xb_col_sum_f32_immediate", "xb_cumsum_rows_f32", "xb_cumsum_cols_f32", "xb_softmax_f32",
+                 "xb_row_sum_f64", "xb_col_amax_f64", "xb_amax_all_f32", "xb_norm_l2_all_f32",
+                 "xb_row_sum_i32_i64", "xb_col_sum_i32_i64"):
+        getattr(lib, name).restype = ctypes.c_double
+    threads = int(lib.xb_threads())
+    model, ncpu = _cpu_info()
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    sz = ctypes.c_size_t
     n = 8  # sample: 8 of the 64 images per pass, passes repeated for ~budget_s
     x, mean, var, gamma, beta = make_inputs(np, n)
     y = np.zeros_like(x)
-    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
-    args = (P(x), P(mean), P(var), P(gamma), P(beta), P(y), n, C, H, W)
-    t1 = f(*args, 1)
-    reps = max(1, int(budget_s / max(t1, 1e-3)))
-    t = f(*args, reps)
-    gbs = algorithmic_bytes(n) * reps / t / 1e9
+    t, reps = _timed(lib.xb_affine_clip_f32, (P(x), P(mean), P(var), P(gamma), P(beta), P(y), n, C, H, W), budget_s)
+    gbs = algorithmic_bytes(n) / t / 1e9
+    per = {}
+
+    def rec(key, nbytes, sec, thr, what):
+        per[key] = {"GBps": round(nbytes / sec / 1e9, 3), "ms": round(sec * 1e3, 3), "threads": thr, "loop": what}
+
+    # config 1: c = a*b + exp(-a*a) (strided loop over rows, threads), s = sum(c, {1}), m = amax(c, {0})
+    rng = np.random.default_rng(0)
+    a = rng.uniform(-1, 1, (2048, 1024))
+    b = rng.uniform(-1, 1, (1024,))
+    c = np.empty_like(a)
+    sec, _ = _timed(lib.xb_mul_add_exp_f64, (P(a), P(b), P(c), 2048, 1024), per_s)
+    rec("mul_add_exp_f64_2048x1024", 2 * a.nbytes + b.nbytes, sec, threads, "strided loop, OpenMP over rows")
+    s1 = np.empty(2048)
+    sec, _ = _timed(lib.xb_row_sum_f64, (P(c), P(s1), 2048, 1024), per_s)
+    rec("sum_axis1_of_c_f64", c.nbytes + s1.nbytes, sec, 1, "lazy fold per row")
+    m1 = np.empty(1024)
+    sec, _ = _timed(lib.xb_col_amax_f64, (P(c), P(m1), 2048, 1024), per_s)
+    rec("amax_axis0_of_c_f64", c.nbytes + m1.nbytes, sec, 1, "immediate: rows folded into the output")
+    del a, c
+    # config 2 int32 variant over the full tensor
+    xi = np.random.default_rng(1).integers(-2**20, 2**20, (N * C * H * W,), dtype=np.int32)
+    yi = np.empty_like(xi)
+    sec, _ = _timed(lib.xb_bitmix_i32, (P(xi), P(yi), sz(xi.size)), per_s)
+    rec("int32_bitmix", 2 * xi.nbytes, sec, threads, "linear loop, OpenMP")
+    del xi, yi
+    # config 3: [8192, 4096]
+    R, K = 8192, 4096
+    X = np.random.default_rng(2).normal(1.0, 1.0, (R, K)).astype(np.float32)
+    o_r, o_c = np.empty(R, np.float32), np.empty(K, np.float32)
+    sec, _ = _timed(lib.xb_row_sum_f32, (P(X), P(o_r), R, K), per_s)
+    rec("sum_axis1_f32", X.nbytes + o_r.nbytes, sec, 1, "lazy fold per row (immediate: the same accumulate)")
+    sl, _ = _timed(lib.xb_col_sum_f32, (P(X), P(o_c), R, K), per_s)
+    si, _ = _timed(lib.xb_col_sum_f32_immediate, (P(X), P(o_c), R, K), per_s)
+    rec("sum_axis0_f32", X.nbytes + o_c.nbytes, min(sl, si), 1,
+        "immediate: rows added into the output" if si <= sl else "lazy fold per column")
+    per["sum_axis0_f32"]["lazy_GBps"] = round((X.nbytes + o_c.nbytes) / sl / 1e9, 3)
+    one = np.empty(1, np.float32)
+    sec, _ = _timed(lib.xb_amax_all_f32, (P(X), P(one), sz(X.size)), per_s)
+    rec("amax_all_f32", X.nbytes + 4, sec, 1, "accumulate over the storage")
+    oned = np.empty(1, np.float64)
+    sec, _ = _timed(lib.xb_norm_l2_all_f32, (P(X), P(oned), sz(X.size)), per_s)
+    rec("norm_l2_all_f32", X.nbytes + 8, sec, 1, "sum of squares in double over the storage")
+    Y = np.empty_like(X)
+    sec, _ = _timed(lib.xb_cumsum_rows_f32, (P(X), P(Y), R, K), per_s)
+    rec("cumsum_axis1_f32", 2 * X.nbytes, sec, 1, "copy, then the recurrence along each row")
+    sec, _ = _timed(lib.xb_cumsum_cols_f32, (P(X), P(Y), R, K), per_s)
+    rec("cumsum_axis0_f32", 2 * X.nbytes, sec, 1, "copy, then the linear sweep one row apart")
+    del X, Y
+    Xi = np.random.default_rng(2).integers(-1000, 1001, (R, K)).astype(np.int32)
+    o64r, o64c = np.empty(R, np.int64), np.empty(K, np.int64)
+    sec, _ = _timed(lib.xb_row_sum_i32_i64, (P(Xi), P(o64r), R, K), per_s)
+    rec("sum_int64_axis1_i32", Xi.nbytes + o64r.nbytes, sec, 1, "lazy fold per row")
+    sec, _ = _timed(lib.xb_col_sum_i32_i64, (P(Xi), P(o64c), R, K), per_s)
+    rec("sum_int64_axis0_i32", Xi.nbytes + o64c.nbytes, sec, 1, "immediate: rows added into the output")
+    del Xi
+    # config 4: row softmax per instance, every 16th instance of the uneven batch
+    from xtensor_amd.workloads import uneven_shapes, uneven_values
+    shapes = uneven_shapes(4096, seed=3)
+    xs = uneven_values(shapes, seed=3)
+    sample = list(range(0, len(shapes), 16))
+    t_all, elems = 0.0, 0
+    for i in sample:
+        r_, c_ = shapes[i]
+        xin = np.ascontiguousarray(xs[i])
+        xo = np.empty_like(xin)
+        tmp = np.empty(r_, np.float32)
+        t_all += lib.xb_softmax_f32(P(xin), P(xo), P(tmp), r_, c_, 1)
+        elems += r_ * c_
+    per["uneven_batch_softmax_f32"] = {"GBps": round(8 * elems / t_all / 1e9, 3),
+                                       "inst_per_s": round(len(sample) / t_all, 1), "threads": 1,
+                                       "loop": f"amax, exp, sum, divide per row; {len(sample)} of 4096 instances"}
     return {"value": round(gbs, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"{n} of {N} images x {reps} passes ({t:.1f} s), scalar stepper-order loop, "
-                      "restatement of xtensor's path (the reference itself is not buildable here)"}
+            "sample": f"{n} of {N} images x {reps} passes ({t * reps:.1f} s), scalar stepper-order loop, "
+                      "restatement of xtensor's path (the reference itself is not buildable here)",
+            "cpu_model": model, "nproc": ncpu, "omp_threads": threads, "per_config": per}
 
 
 def make_uneven_batch(np, torch, xt, n_inst=4096):
@@ -263,7 +367,28 @@ def reductions(np, torch, xt, steps=12, copies=3):
                                "max_abs_err_over_rowsum_abs": float(np.max(np.abs(got - ref) /
                                                                     np.abs(X64).sum(1, keepdims=True))),
                                "kernel": xt.last_kernel()}
-    del css, pcs, dX, dXb, dXi
+    del css, pcs
+
+    def scan(name, views, ref, axis):
+        """a scan plan per input copy; error against a float64 scan relative
+        to max(|ref|, 1)"""
+        outs = [xt.empty(v.shape, "float32") for v in views]
+        plans = [xt.plan(o, xt.Scan("cumsum", v, axis)) for o, v in zip(outs, views)]
+        _, skms = time_plans(torch, plans, steps, len(plans))
+        kname = xt.last_kernel()
+        got = outs[0].to_numpy().astype(np.float64)
+        out[name] = {"kernel_ms": round(skms, 4), "GBps": round(2 * nb / (skms * 1e-3) / 1e9, 1),
+                     "max_rel_err_vs_fp64": float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0))),
+                     "kernel": kname}
+
+    # the scan axis strided: cumsum(X, 0) and cumsum(transpose(X), 1) (tiled
+    # kernels), and the flattened cumsum(X) (one row of 2^25 elements: the
+    # look-back kernel; planned as the scan of the 1-d view of X's storage)
+    scan("cumsum_axis0_f32", dX, np.cumsum(X64, axis=0), 0)
+    scan("cumsum_axis1_transposed_f32", [d.T for d in dX], np.cumsum(X64.T, axis=1), 1)
+    scan("cumsum_flat_f32", [xt.DeviceArray(d.storage, "float32", (R * Cc,), None, 0) for d in dX],
+         np.cumsum(X64.ravel()), 0)
+    del dX, dXb, dXi
     # the broadcast expression of 2048 x 1024 float64 and its reductions
     r1 = np.random.default_rng(0)
     a = r1.uniform(-1, 1, (2048, 1024))

@@ -14,12 +14,19 @@
 //    outer rows x inner contiguous run; threads over rows (OpenMP here in
 //    place of TBB).
 //  * row / column reductions: xreducer_stepper::aggregate_impl, a sequential
-//    fold per output (reducers/xreducer.hpp:1801-1828), single-threaded.
+//    fold per output (reducers/xreducer.hpp:1801-1828), single-threaded; and
+//    evaluation_strategy::immediate (reducers/xreducer.hpp:289-565): a
+//    std::accumulate per contiguous run, or a std::transform that adds whole
+//    rows into the output when the reduced axis is strided, single-threaded.
+//  * accumulators: a copy, then the linear sweep res[k + s] = res[k] + res[k + s]
+//    with s the axis stride (reducers/xaccumulator.hpp:215-297).
 // Built with -O3 -march=native -ffp-contract=off.
 #include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+
+#include <omp.h>
 
 namespace {
 inline double now() {
@@ -126,6 +133,118 @@ double xb_softmax_f32(const float* x, float* out, float* tmp, int rows, int cols
             tmp[i] = s;
         }
     return now() - t0;
+}
+
+// ---- reductions and scans of [rows, cols] arrays, single-threaded --------
+
+// row sums of a float64 array (config 1's s = sum(c, {1}), lazy fold)
+double xb_row_sum_f64(const double* x, double* out, int rows, int cols, int reps) {
+    double t0 = now();
+    for (int r = 0; r < reps; ++r)
+        for (int i = 0; i < rows; ++i) {
+            double acc = 0.0;
+            const double* p = x + size_t(i) * cols;
+            for (int j = 0; j < cols; ++j) acc = acc + p[j];
+            out[i] = acc;
+        }
+    return now() - t0;
+}
+
+// column maxima of a float64 array (config 1's m = amax(c, {0})), immediate
+// form: the first row copied, then every row folded in with maximum's select
+double xb_col_amax_f64(const double* x, double* out, int rows, int cols, int reps) {
+    double t0 = now();
+    for (int r = 0; r < reps; ++r) {
+        for (int j = 0; j < cols; ++j) out[j] = xsel(-1.7976931348623157e308 > x[j], -1.7976931348623157e308, x[j]);
+        for (int i = 1; i < rows; ++i) {
+            const double* p = x + size_t(i) * cols;
+            for (int j = 0; j < cols; ++j) out[j] = xsel(out[j] > p[j], out[j], p[j]);
+        }
+    }
+    return now() - t0;
+}
+
+// column sums of a float32 array, immediate form (rows added into the output)
+double xb_col_sum_f32_immediate(const float* x, float* out, int rows, int cols, int reps) {
+    double t0 = now();
+    for (int r = 0; r < reps; ++r) {
+        for (int j = 0; j < cols; ++j) out[j] = 0.f + x[j];
+        for (int i = 1; i < rows; ++i) {
+            const float* p = x + size_t(i) * cols;
+            for (int j = 0; j < cols; ++j) out[j] = out[j] + p[j];
+        }
+    }
+    return now() - t0;
+}
+
+// amax over every element (std::accumulate with maximum)
+double xb_amax_all_f32(const float* x, float* out, size_t n, int reps) {
+    double t0 = now();
+    for (int r = 0; r < reps; ++r) {
+        float acc = -3.402823466e38f;
+        for (size_t k = 0; k < n; ++k) acc = xsel(acc > x[k], acc, x[k]);
+        out[0] = acc;
+    }
+    return now() - t0;
+}
+
+// norm_l2 over every element: sqrt of the sum of squares in double
+double xb_norm_l2_all_f32(const float* x, double* out, size_t n, int reps) {
+    double t0 = now();
+    for (int r = 0; r < reps; ++r) {
+        double acc = 0.0;
+        for (size_t k = 0; k < n; ++k) acc = acc + double(x[k]) * double(x[k]);
+        out[0] = std::sqrt(acc);
+    }
+    return now() - t0;
+}
+
+// cumsum along the strided axis 0: copy, then the linear sweep
+// out[k + cols] = out[k] + out[k + cols]
+double xb_cumsum_cols_f32(const float* x, float* out, int rows, int cols, int reps) {
+    double t0 = now();
+    for (int r = 0; r < reps; ++r) {
+        std::memcpy(out, x, sizeof(float) * size_t(rows) * cols);
+        const size_t n = size_t(rows - 1) * cols;
+        for (size_t k = 0; k < n; ++k) out[k + cols] = out[k] + out[k + cols];
+    }
+    return now() - t0;
+}
+
+// sum<int64_t> of an int32 array over axis 1 (lazy fold) and axis 0 (immediate rows)
+double xb_row_sum_i32_i64(const int32_t* x, int64_t* out, int rows, int cols, int reps) {
+    double t0 = now();
+    for (int r = 0; r < reps; ++r)
+        for (int i = 0; i < rows; ++i) {
+            int64_t acc = 0;
+            const int32_t* p = x + size_t(i) * cols;
+            for (int j = 0; j < cols; ++j) acc = acc + p[j];
+            out[i] = acc;
+        }
+    return now() - t0;
+}
+
+double xb_col_sum_i32_i64(const int32_t* x, int64_t* out, int rows, int cols, int reps) {
+    double t0 = now();
+    for (int r = 0; r < reps; ++r) {
+        for (int j = 0; j < cols; ++j) out[j] = int64_t(0) + x[j];
+        for (int i = 1; i < rows; ++i) {
+            const int32_t* p = x + size_t(i) * cols;
+            for (int j = 0; j < cols; ++j) out[j] = out[j] + p[j];
+        }
+    }
+    return now() - t0;
+}
+
+// threads an OpenMP loop of this library runs on
+int xb_threads(void) {
+    int n = 1;
+#pragma omp parallel
+    {
+#pragma omp single
+        n = omp_get_num_threads();
+    }
+    return n;
 }
 
 }  // extern "C"
