@@ -283,15 +283,15 @@ def test_tiled_scan_store_conversion_and_plans():
 
 @pytest.mark.parametrize("dtype", ["float32", "float64", "int32", "int64"])
 def test_lookback_long_rows(dtype):
-    """Few rows much longer than one chunk (4096 elements of 4-byte types,
-    2048 of 8-byte types) run the single-pass look-back scan: the flattened
+    """Few rows much longer than one chunk (16384 elements of 4-byte types,
+    8192 of 8-byte types) run the single-pass look-back scan: the flattened
     cumsum of a large array, and cumsum along axis 1 of a few long rows.
     Lengths around the chunk edges and one past 2^25.  Integers exact; floats
     against a float64 scan (rel 2e-7 for float32, 1e-14 for float64); the
     float64 and int64 cases use the flag + value publication, the others the
     packed word."""
     rng = np.random.default_rng(41)
-    for n in [4097, 8192, 8193, 300_001, (1 << 25) + 7]:
+    for n in [16385, 32768, 32769, 300_001, (1 << 25) + 7]:
         x = _tiled_input(rng, (n,), dtype, "cumsum")
         if np.dtype(dtype).kind != "f":
             x = (x % 7 - 3).astype(dtype)

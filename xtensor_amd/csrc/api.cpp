@@ -865,6 +865,9 @@ void xg_debug_jit(int mode) { jit_set_mode(mode); }
 // Measurement aid: waves per CU the reduction work split aims at (0: default).
 void xg_debug_reduce_waves(int per_cu) { set_reduce_waves_per_cu(per_cu); }
 
+// Measurement aid: workgroups per CU the tiled scans aim at (0: default).
+void xg_debug_scan_blocks_per_cu(int per_cu) { set_scan_blocks_per_cu(per_cu); }
+
 // Counters of the run-time compiler: compiles, disk-cache hits, in-process
 // hits, failures (out[0..3]); returns the last failure message.
 const char* xg_debug_jit_stats(int64_t* out) {

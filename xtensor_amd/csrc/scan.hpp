@@ -37,6 +37,10 @@ struct TileScan {
     int32_t nt_in, nt_out;
 };
 
+// Segments of 64 lanes x 16 bytes per wave in one chunk of the look-back scan
+// (a chunk is 4 waves x kLbSegments segments).
+constexpr int kLbSegments = 16;
+
 // Kernel argument of the look-back scan of long contiguous rows
 // (scan_lookback.hpp); pointers into the workspace filled in at launch.
 struct LbScan {
@@ -82,6 +86,10 @@ struct ScanPlan {
     int32_t lb_packed;  // its publications are packed 64-bit words
     LbScan lb;
 };
+
+// Measurement aid: workgroups per compute unit the tiled scans' chunking
+// aims at, for plans made after the call (0: the default, 8).
+void set_scan_blocks_per_cu(int n);
 
 // The direct rows kernels apply: rank <= 2 rows layout with 32-bit offsets,
 // a program that is one load of a leaf of the accumulation input type, the

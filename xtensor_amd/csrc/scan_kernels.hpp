@@ -325,6 +325,7 @@ __global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p)
 
 }  // namespace xg
 
+#include "scan_tiled.hpp"
 #include "scan_lookback.hpp"
 
 namespace xg {
@@ -332,12 +333,6 @@ namespace xg {
 // elements of one wave's tile in the direct rows scan (rows up to this
 // length take one wave each)
 template <class T> constexpr uint32_t scan_wave_tile() { return 64u * uint32_t(16 / sizeof(T)) * 4u; }
-
-}  // namespace xg
-
-#include "scan_tiled.hpp"
-
-namespace xg {
 
 using SFn = void (*)(const ScanPlan);
 using TFn = void (*)(const TileScan);

@@ -12,9 +12,9 @@
 //   * the workgroup loads its chunk, scans it locally (the tile scheme of the
 //     direct rows kernel) and publishes the chunk total as an aggregate;
 //   * one wave looks back over the 64 preceding chunks at once, waits until
-//     each has published something, and merges them back to the nearest
-//     chunk that has published its inclusive prefix (the row's first chunk
-//     always has); then it publishes its own inclusive prefix;
+//     those nearer than the nearest published inclusive prefix have published
+//     their totals, and merges them back to that prefix (the row's first
+//     chunk always publishes one); then it publishes its own inclusive prefix;
 //   * every element's result is that exclusive prefix merged with its local
 //     prefix.
 // Publication: for accumulators that fit 64 bits with a flag (float32 sums
@@ -127,7 +127,10 @@ __device__ __forceinline__ Acc<AK, OP> lb_lookback(const LbScan& p, uint32_t fir
         int spins = 0;
         while (true) {
             if (f == 0) f = lb_poll<AK, OP>(p, first + uint32_t(j), &v);
-            if (__ballot(f == 0) == 0) break;
+            // wait only for the chunks nearer than the nearest published prefix
+            const uint64_t pm0 = __ballot(valid && f == kLbPrefix);
+            const int lim = pm0 ? __builtin_ctzll(pm0) : 64;
+            if (__ballot(f == 0 && lane < lim) == 0) break;
             if (++spins >= kSpinLimit) {  // bounded: count the timeout, take the identity
                 if (f == 0) {
                     v = id;
@@ -155,13 +158,15 @@ __device__ __forceinline__ Acc<AK, OP> lb_lookback(const LbScan& p, uint32_t fir
     return excl;
 }
 
-// One chunk of TILE = 4 waves x U segments x 64 * V elements per workgroup.
+// One chunk of TILE = 4 waves x U segments x 64 * V elements per workgroup
+// (U = kLbSegments: 16384 float32 elements; with 4096-element chunks the
+// chain of look-backs and the ticket counter measured slower on MI355X).
 template <int AK, int OP, bool NATIVE>
 __global__ __launch_bounds__(256) void scan_lookback_kernel(const LbScan p) {
     using A = Acc<AK, OP>;
     using T = typename ScanIn<AK>::T;
     constexpr int V = int(16 / sizeof(T));
-    constexpr int U = 4;
+    constexpr int U = kLbSegments;
     constexpr uint32_t SEG = 64u * V;
     __shared__ uint32_t s_ticket;
     __shared__ Store wtot[4];
@@ -183,21 +188,27 @@ __global__ __launch_bounds__(256) void scan_lookback_kernel(const LbScan p) {
     A id;
     id.init(p.identity);
     const uint32_t wbase = c * p.tile + uint32_t(wave) * SEG * U;
+    // a whole chunk (all but a row's last) loads without a branch between the loads
+    const bool whole = (c + 1u) * p.tile <= p.len;  // uniform in the block
     T x[U][V];
     int nv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const uint32_t j = wbase + uint32_t(u) * SEG + uint32_t(lane * V);
         nv[u] = j < p.len ? min(V, int(p.len - j)) : 0;
-        if (nv[u] == V) {
-            Reg r[V];
-            if (nt_in) load_vec<V, true>(reinterpret_cast<const char*>(in + j), int(sizeof(T)), ScanInDtype<T>::value, r);
-            else load_vec<V>(reinterpret_cast<const char*>(in + j), int(sizeof(T)), ScanInDtype<T>::value, r);
+    }
+    if (whole && nt_in) {
 #pragma unroll
-            for (int v = 0; v < V; ++v) x[u][v] = reg_get<T>(r[v]);
-        } else {
+        for (int u = 0; u < U; ++u) load_row<V, true>(in + wbase + uint32_t(u) * SEG + uint32_t(lane * V), x[u]);
+    } else if (whole) {
 #pragma unroll
-            for (int v = 0; v < V; ++v) x[u][v] = v < nv[u] ? in[j + uint32_t(v)] : T(0);
+        for (int u = 0; u < U; ++u) load_row<V, false>(in + wbase + uint32_t(u) * SEG + uint32_t(lane * V), x[u]);
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t j = wbase + uint32_t(u) * SEG + uint32_t(lane * V);
+            if (nv[u] == V) load_row<V, false>(in + j, x[u]);
+            else load_row_partial<V>(in + j, true, nv[u], x[u]);
         }
     }
     // wave-local exclusive prefixes of the segments, and the wave's total

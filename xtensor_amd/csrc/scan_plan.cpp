@@ -65,6 +65,8 @@ void scan_result(const xg_expr* e, const xg_scan_desc* s, int* dtype, Shape* sha
 
 namespace {
 
+int g_tile_blocks_per_cu = 8;  // workgroups per compute unit the tiled scans aim at
+
 // element type the direct and tiled kernels read, per accumulator kind
 int scan_in_dtype(int acc) {
     static const int t[AK_NUM] = {XG_FLOAT32, XG_FLOAT64, XG_FLOAT32, XG_FLOAT64,
@@ -126,7 +128,7 @@ void plan_tiled(ScanPlan* pp, int cus) {
     t.ncols = uint32_t(t.ntiles * cols);
     // about four workgroups per compute unit, chunks of whole sub-tiles
     const uint64_t per_chunk = uint64_t(t.outer) * t.ntiles;
-    const uint64_t target = uint64_t(std::max(cus, 1)) * 4;
+    const uint64_t target = uint64_t(std::max(cus, 1)) * uint64_t(g_tile_blocks_per_cu);
     uint64_t want = (target + per_chunk - 1) / per_chunk;
     const uint64_t max_chunks = std::min<uint64_t>(256, (uint64_t(t.len) + sub - 1) / sub);
     want = std::max<uint64_t>(1, std::min(want, max_chunks));
@@ -147,14 +149,14 @@ void plan_tiled(ScanPlan* pp, int cus) {
 }
 
 // Rows too few to fill the GPU with a workgroup each, and longer than one
-// chunk, run the look-back kernel: chunks of 4 waves x 4 segments x 64 lanes
-// x 16 bytes (4096 float32 elements).
+// chunk, run the look-back kernel: chunks of 4 waves x kLbSegments segments
+// x 64 lanes x 16 bytes (16384 float32 elements).
 void plan_lookback(ScanPlan* pp, int cus) {
     ScanPlan& p = *pp;
     p.lookback = 0;
     if (!scan_direct_ok(p)) return;
     const int in_dt = scan_in_dtype(p.acc);
-    const uint32_t tile = 16u * 64u * uint32_t(16 / dtype_size(in_dt));
+    const uint32_t tile = 4u * uint32_t(kLbSegments) * 64u * uint32_t(16 / dtype_size(in_dt));
     if (p.len <= tile || uint64_t(p.rows) * 2 >= uint64_t(std::max(cus, 1)) * 4) return;
     const uint64_t cpr = (uint64_t(p.len) + tile - 1) / tile;
     if (cpr * p.rows >= (uint64_t(1) << 31)) return;
@@ -181,6 +183,8 @@ void plan_lookback(ScanPlan* pp, int cus) {
 size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 
 }  // namespace
+
+void set_scan_blocks_per_cu(int n) { g_tile_blocks_per_cu = n > 0 ? n : 8; }
 
 bool scan_direct_ok(const ScanPlan& p) {
     if (p.layout != SL_ROWS || p.index64 || p.rank < 1 || p.rank > 2 || p.acc < 0 || p.acc >= AK_NUM) return false;

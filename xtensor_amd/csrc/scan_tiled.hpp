@@ -33,12 +33,26 @@
 
 namespace xg {
 
+// V elements of type T (16 bytes, aligned) from ptr
+template <int V, bool NT, class T> __device__ __forceinline__ void load_row(const T* ptr, T (&x)[V]) {
+    Reg q[V];
+    load_vec<V, NT>(reinterpret_cast<const char*>(ptr), int(sizeof(T)), ScanInDtype<T>::value, q);
+#pragma unroll
+    for (int v = 0; v < V; ++v) x[v] = reg_get<T>(q[v]);
+}
+
+// the same V elements, each only if v < nv and `ok` (else 0): the ragged edges
+template <int V, class T> __device__ __forceinline__ void load_row_partial(const T* ptr, bool ok, int nv, T (&x)[V]) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) x[v] = (ok && v < nv) ? ptr[v] : T(0);
+}
+
 template <int AK, int OP>
 __global__ __launch_bounds__(256) void scan_tiles_totals_kernel(const TileScan p) {
     using A = Acc<AK, OP>;
     using T = typename ScanIn<AK>::T;
     constexpr int V = int(16 / sizeof(T));
-    constexpr int U = 4;  // rows in flight per wave
+    constexpr int U = 8;  // rows in flight per wave
     constexpr int COLS = 64 * V;
     __shared__ Store part[4][COLS];
     const int lane = threadIdx.x & 63;
@@ -58,25 +72,25 @@ __global__ __launch_bounds__(256) void scan_tiles_totals_kernel(const TileScan p
     A acc[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = id;
+    // whole tiles (every lane V columns): the U rows' loads issue together,
+    // rows past the chunk clamped to its last row and their values skipped
+    const bool whole = (tile + 1u) * uint32_t(COLS) <= p.inner;  // uniform in the block
     for (uint32_t r = r0 + uint32_t(wave); r < r1; r += 4u * U) {
         T x[U][V];
+        if (whole) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t row = r + 4u * uint32_t(u);
-            const T* ptr = base + int64_t(row) * p.in_s_len;
-            if (row < r1 && nv == V) {  // row: uniform in the wave
-                Reg q[V];
-                load_vec<V>(reinterpret_cast<const char*>(ptr), int(sizeof(T)), ScanInDtype<T>::value, q);
+            for (int u = 0; u < U; ++u)
+                load_row<V, false>(base + int64_t(min(r + 4u * uint32_t(u), r1 - 1)) * p.in_s_len, x[u]);
+        } else {
 #pragma unroll
-                for (int v = 0; v < V; ++v) x[u][v] = reg_get<T>(q[v]);
-            } else {
-#pragma unroll
-                for (int v = 0; v < V; ++v) x[u][v] = (row < r1 && v < nv) ? ptr[v] : T(0);
+            for (int u = 0; u < U; ++u) {
+                const uint32_t row = r + 4u * uint32_t(u);
+                load_row_partial<V>(base + int64_t(min(row, r1 - 1)) * p.in_s_len, row < r1, nv, x[u]);
             }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (r + 4u * uint32_t(u) >= r1) continue;
+            if (r + 4u * uint32_t(u) >= r1) continue;  // uniform in the wave
 #pragma unroll
             for (int v = 0; v < V; ++v)
                 if (v < nv) acc[v].add(reg_put<T>(x[u][v]));
@@ -183,22 +197,25 @@ __global__ __launch_bounds__(256) void scan_tiles_kernel(const TileScan p) {
         carry[v] = id;
         if (p.nchunks > 1) carry[v].load(p.part[(int64_t(o) * p.nchunks + chunk) * p.ncols + col + uint32_t(v)]);
     }
+    const bool whole = (tile + 1u) * uint32_t(COLS) <= p.inner;  // uniform in the block
     for (uint32_t sub = r0; sub < r1; sub += SUB) {
         const uint32_t rw0 = sub + uint32_t(wave * RW);
         T x[RW][V];
+        // whole tiles: all RW rows' loads issue together (rows past the chunk
+        // clamped to its last row; their values are never used)
+        if (whole && nt_in) {
 #pragma unroll
-        for (int i = 0; i < RW; ++i) {
-            const uint32_t row = rw0 + uint32_t(i);
-            const T* ptr = base + int64_t(row) * p.in_s_len;
-            if (row < r1 && nv == V) {  // row: uniform in the wave
-                Reg q[V];
-                if (nt_in) load_vec<V, true>(reinterpret_cast<const char*>(ptr), int(sizeof(T)), ScanInDtype<T>::value, q);
-                else load_vec<V>(reinterpret_cast<const char*>(ptr), int(sizeof(T)), ScanInDtype<T>::value, q);
+            for (int i = 0; i < RW; ++i)
+                load_row<V, true>(base + int64_t(min(rw0 + uint32_t(i), r1 - 1)) * p.in_s_len, x[i]);
+        } else if (whole) {
 #pragma unroll
-                for (int v = 0; v < V; ++v) x[i][v] = reg_get<T>(q[v]);
-            } else {
+            for (int i = 0; i < RW; ++i)
+                load_row<V, false>(base + int64_t(min(rw0 + uint32_t(i), r1 - 1)) * p.in_s_len, x[i]);
+        } else {
 #pragma unroll
-                for (int v = 0; v < V; ++v) x[i][v] = (row < r1 && v < nv) ? ptr[v] : T(0);
+            for (int i = 0; i < RW; ++i) {
+                const uint32_t row = rw0 + uint32_t(i);
+                load_row_partial<V>(base + int64_t(min(row, r1 - 1)) * p.in_s_len, row < r1, nv, x[i]);
             }
         }
         // this wave's totals, then the prefix of the waves before it
