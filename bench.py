@@ -550,8 +550,9 @@ def main():
                     help="profiling pass for rocprofv3: the uneven softmax batch, one launch and 3 graph replays")
     ap.add_argument("--profile-reduce", action="store_true",
                     help="profiling pass for rocprofv3: a 256 MiB float32 copy (counter calibration), then "
-                         "sum axis 0 / axis 1, amax over all, cumsum axis 1 of f32 [8192, 4096], 3 launches "
-                         "each over three input copies")
+                         "every reduction and scan of the bench line on [8192, 4096] (float32, its bf16 copy, "
+                         "its transpose, int32) and the 2048 x 1024 float64 expression with its row sum and "
+                         "column amax, 3 launches each over three input copies")
     ap.add_argument("--profile", action="store_true",
                     help="profiling pass for rocprofv3: a 256 MiB float32 copy (counter calibration, 3 "
                          "launches) and the workload kernel as in the timed loop (6 + 12 launches rotating "
@@ -587,14 +588,45 @@ def main():
             cp.run()
         X = np.random.default_rng(2).normal(1.0, 1.0, (8192, 4096)).astype(np.float32)
         dX = [xt.from_numpy(X) for _ in range(3)]
+        dXb = [xt.from_numpy(X, "bfloat16") for _ in range(3)]
+        dXi = [xt.from_numpy(np.random.default_rng(2).integers(-1000, 1001, (8192, 4096), dtype=np.int32))
+               for _ in range(3)]
+        R_, C_ = X.shape
+        flat = lambda d: xt.DeviceArray(d.storage, "float32", (R_ * C_,), None, 0)  # noqa: E731
         names = {}
-        for name, make, oshape in (("sum_axis0", lambda d: xt.sum(d, [0]), (4096,)),
-                                   ("sum_axis1", lambda d: xt.sum(d, [1]), (8192,)),
-                                   ("amax_all", lambda d: xt.amax(d), ()),
-                                   ("cumsum_axis1", lambda d: xt.Scan("cumsum", d, 1), (8192, 4096))):
-            outs = [xt.empty(oshape, "float32") for _ in dX]
-            plans = [xt.plan(o, make(d)) for o, d in zip(outs, dX)]
+        for name, srcs, make, oshape, odt in (
+                ("sum_axis0_f32", dX, lambda d: xt.sum(d, [0]), (4096,), "float32"),
+                ("sum_axis1_f32", dX, lambda d: xt.sum(d, [1]), (8192,), "float32"),
+                ("mean_axis0_f32", dX, lambda d: xt.mean(d, [0]), (4096,), "float64"),
+                ("mean_axis1_f32", dX, lambda d: xt.mean(d, [1]), (8192,), "float64"),
+                ("amax_all_f32", dX, lambda d: xt.amax(d), (), "float32"),
+                ("norm_l2_all_f32", dX, lambda d: xt.norm_l2(d), (), "float64"),
+                ("sum_axis0_transposed_f32", dX, lambda d: xt.sum(d.T, [0]), (8192,), "float32"),
+                ("sum_axis1_transposed_f32", dX, lambda d: xt.sum(d.T, [1]), (4096,), "float32"),
+                ("sum_axis0_bf16", dXb, lambda d: xt.sum(d, [0]), (4096,), "float32"),
+                ("sum_axis1_bf16", dXb, lambda d: xt.sum(d, [1]), (8192,), "float32"),
+                ("sum_int64_axis0_i32", dXi, lambda d: xt.sum(d, [0], dtype="int64"), (4096,), "int64"),
+                ("sum_int64_axis1_i32", dXi, lambda d: xt.sum(d, [1], dtype="int64"), (8192,), "int64"),
+                ("cumsum_axis1_f32", dX, lambda d: xt.Scan("cumsum", d, 1), (8192, 4096), "float32"),
+                ("cumsum_axis0_f32", dX, lambda d: xt.Scan("cumsum", d, 0), (8192, 4096), "float32"),
+                ("cumsum_axis1_transposed_f32", dX, lambda d: xt.Scan("cumsum", d.T, 1), (4096, 8192), "float32"),
+                ("cumsum_flat_f32", dX, lambda d: xt.Scan("cumsum", flat(d), 0), (R_ * C_,), "float32")):
+            outs = [xt.empty(oshape, odt) for _ in srcs]
+            plans = [xt.plan(o, make(d)) for o, d in zip(outs, srcs)]
             for p in plans:
+                p.run()
+            names[name] = xt.last_kernel()
+            torch.cuda.synchronize()
+            del outs, plans
+        r1 = np.random.default_rng(0)
+        a = xt.from_numpy(r1.uniform(-1, 1, (2048, 1024)))
+        b = xt.from_numpy(r1.uniform(-1, 1, 1024))
+        c = xt.empty((2048, 1024), "float64")
+        s1, m1 = xt.empty((2048,), "float64"), xt.empty((1024,), "float64")
+        for name, p in (("mul_add_exp_f64_2048x1024", xt.plan(c, a * b + xt.exp(-a * a))),
+                        ("sum_axis1_of_c_f64", xt.plan(s1, xt.sum(c, [1]))),
+                        ("amax_axis0_of_c_f64", xt.plan(m1, xt.amax(c, [0])))):
+            for _ in range(3):
                 p.run()
             names[name] = xt.last_kernel()
             torch.cuda.synchronize()

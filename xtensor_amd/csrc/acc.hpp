@@ -38,6 +38,10 @@ template <int OP> struct Acc<AK_F32C, OP> {
     __device__ Acc shfl_idx(int src) const { Acc r; r.s = __shfl(s, src); r.c = __shfl(c, src); return r; }
     __device__ Store store() const { return Store{__float_as_uint(s), __float_as_uint(c)}; }
     __device__ void load(const Store& st) { s = __uint_as_float(uint32_t(st.a)); c = __uint_as_float(uint32_t(st.b)); }
+    // the accumulator as it sits in LDS (8 bytes, not a 16-byte Store)
+    using Lds = float2;
+    __device__ Lds lds() const { return float2{s, c}; }
+    __device__ void from_lds(const Lds& v) { s = v.x; c = v.y; }
     __device__ Reg value(int vdt) const {
         if (vdt == XG_FLOAT32) return from_f32(s + c);
         return from_f64(double(s) + double(c));
@@ -66,6 +70,9 @@ template <int OP> struct Acc<AK_F64C, OP> {
     __device__ Acc shfl_idx(int src) const { Acc r; r.s = __shfl(s, src); r.c = __shfl(c, src); return r; }
     __device__ Store store() const { return Store{uint64_t(__double_as_longlong(s)), uint64_t(__double_as_longlong(c))}; }
     __device__ void load(const Store& st) { s = __longlong_as_double(int64_t(st.a)); c = __longlong_as_double(int64_t(st.b)); }
+    using Lds = double2;
+    __device__ Lds lds() const { return double2{s, c}; }
+    __device__ void from_lds(const Lds& v) { s = v.x; c = v.y; }
     __device__ Reg value(int) const { return from_f64(s + c); }
 };
 
@@ -119,6 +126,9 @@ template <int AK, int OP> struct Acc {
     __device__ Acc shfl_idx(int src) const { Acc r; r.v = __shfl(v, src); return r; }
     __device__ Store store() const { return Store{uint64_t(reg_put<T>(v)), 0}; }
     __device__ void load(const Store& st) { v = reg_get<T>(st.a); }
+    using Lds = T;
+    __device__ Lds lds() const { return v; }
+    __device__ void from_lds(const Lds& x) { v = x; }
     __device__ Reg value(int vdt) const { return cvt(reg_put<T>(v), AccType<AK>::dt, vdt); }
 };
 

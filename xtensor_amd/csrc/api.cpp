@@ -100,6 +100,28 @@ void* workspace(size_t bytes, hipStream_t stream) {
     return b.ptr;
 }
 
+namespace {
+std::map<WsKey, uint32_t*> g_counters;
+}  // namespace
+
+// Arrival counters of the column reductions that merge their own splits:
+// one zeroed block of kReduceCounters words per (device, stream), cleared
+// once when allocated (every launch leaves them zero again).
+uint32_t* reduce_counters(hipStream_t stream) {
+    int dev = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    uint32_t*& c = g_counters[WsKey{dev, reinterpret_cast<void*>(stream)}];
+    if (!c) {
+        void* m = nullptr;
+        if (hipMalloc(&m, size_t(kReduceCounters) * sizeof(uint32_t)) != hipSuccess)
+            raise(XG_OUT_OF_MEMORY, "counter allocation failed");
+        check_hip(hipMemsetAsync(m, 0, size_t(kReduceCounters) * sizeof(uint32_t), stream), "hipMemsetAsync");
+        c = static_cast<uint32_t*>(m);
+    }
+    return c;
+}
+
 // base address of a sizing pass's block: 256-byte aligned like a real one,
 // so that plans made against it choose the same vector accesses
 constexpr uintptr_t kSizingBase = uintptr_t(1) << 40;
@@ -235,6 +257,7 @@ void assign_launch(const xg_expr* e, const xg_tensor* out, hipStream_t st, const
 // Launch a planned reduction with its split partials from the arena.
 void run_reduce(RedPlan& p, int V, hipStream_t st, Arena& a, const char** used) {
     p.partials = a.take(reduce_workspace_bytes(p));
+    p.counters = a.counters;
     if (!a.measure && p.n_out) check_hip(launch_reduce(p, V, st, used), "reduction launch");
 }
 
@@ -411,9 +434,11 @@ bool reduce_in_pieces(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor
     for (int64_t q = 0; q < P.k_pieces; ++q) {
         RedPlan p = plan_piece(q, &V);
         p.partials = reduce_workspace_bytes(p) ? part : nullptr;
+        p.counters = a.counters;
         check_hip(launch_reduce(p, V, st, used), "reduction launch");
     }
     pm.partials = reduce_workspace_bytes(pm) ? part : nullptr;
+    pm.counters = a.counters;
     check_hip(launch_reduce(pm, Vm, st, used), "reduction launch");
     return true;
 }
@@ -553,7 +578,7 @@ void reduce_top(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out,
 }  // namespace
 
 void xg::reduce_call(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t st, BlockSource src,
-                     void* ctx) {
+                     void* ctx, bool counters) {
     const char* used = nullptr;
     Arena sizing;
     reduce_top(e, r, out, st, sizing, &used);
@@ -561,6 +586,7 @@ void xg::reduce_call(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor*
     a.measure = false;
     a.cap = sizing.used;
     a.base = sizing.used ? src(sizing.used, st, ctx) : nullptr;
+    a.counters = counters ? reduce_counters(st) : nullptr;
     used = nullptr;
     reduce_top(e, r, out, st, a, &used);
     note_kernel(used ? used : "empty");
@@ -614,7 +640,7 @@ xg_status xg_reduce_info(const xg_expr* e, const xg_reduce_desc* r, int32_t* dty
 }
 
 xg_status xg_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, xg_stream_t stream) {
-    return guard([&] { reduce_call(e, r, out, reinterpret_cast<hipStream_t>(stream), stream_workspace, nullptr); });
+    return guard([&] { reduce_call(e, r, out, reinterpret_cast<hipStream_t>(stream), stream_workspace, nullptr, true); });
 }
 
 xg_status xg_scan_info(const xg_expr* e, const xg_scan_desc* s, int32_t* dtype, int32_t* ndim, int64_t* shape) {
@@ -801,6 +827,7 @@ xg_status xg_plan_execute(xg_plan_t p, void* ws, xg_stream_t stream) {
         } else if (p->kind == 1) {
             if (p->red.n_out) {
                 p->red.partials = p->ws ? (ws ? ws : workspace(p->ws, st)) : nullptr;
+                p->red.counters = ws ? nullptr : reduce_counters(st);
                 check_hip(launch_reduce(p->red, p->V, st, &used), "reduction launch");
             }
         } else {
@@ -834,6 +861,7 @@ xg_status xg_plan_execute_many(const xg_plan_t* plans, int64_t n, xg_stream_t st
             const char* used = nullptr;
             if (p->kind == 1 && p->red.n_out) {
                 p->red.partials = p->ws ? workspace(p->ws, st) : nullptr;
+                p->red.counters = reduce_counters(st);
                 check_hip(launch_reduce(p->red, p->V, st, &used), "reduction launch");
             } else if (p->kind == 2) {
                 check_hip(launch_scan(p->scan, p->V, p->ws ? workspace(p->ws, st) : nullptr, st, &used),

@@ -71,7 +71,7 @@ xg_status xg_reduce_noalloc(const xg_expr* e, const xg_reduce_desc* r, const xg_
         scratch_setup();
         const int prev = jit_set_thread_loaded_only(1);
         try {
-            reduce_call(e, r, out, reinterpret_cast<hipStream_t>(stream), scratch_block, nullptr);
+            reduce_call(e, r, out, reinterpret_cast<hipStream_t>(stream), scratch_block, nullptr, false);
         } catch (...) {
             jit_set_thread_loaded_only(prev);
             throw;

@@ -76,6 +76,12 @@ struct RedPlan {
     char* out_ptr;
     int64_t out_stride[kMaxRank];  // over the kept dims in normalised order (fast last for RL_COLS)
     void* partials;     // workspace: splits * n_out accumulators of 16 bytes
+    // cols fast kernel with splits: one arrival counter per tile, zero
+    // between launches; the last split of a tile to finish merges the tile's
+    // partials (with `combine_group` lanes per output, as the combine kernel
+    // would).  Null: a separate combine kernel merges them.
+    uint32_t* counters;
+    int32_t combine_group;
     LeafView leaves[kMaxLeaves];
     Program prog;       // element value mapped and converted to the accumulation type
     // Compact form for the compiled fast kernels: normalised rank <= 2 and
@@ -111,6 +117,10 @@ RedFinish reduce_finish(const xg_expr* e, const xg_reduce_desc* r);
 #if !defined(__HIPCC_RTC__)  // host API (absent from run-time compiled sources)
 hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** used);
 #endif
+
+// Arrival counters the cols fast kernel can merge its splits with: zeroed
+// device memory of kReduceCounters words, left zeroed by every launch.
+constexpr uint32_t kReduceCounters = 4096;
 
 size_t reduce_workspace_bytes(const RedPlan& p);
 

@@ -173,7 +173,7 @@ template <int V, int NR, class S, bool I64, int AK, int OP, int kUnroll>
 __global__ __launch_bounds__(256) void reduce_cols_kernel(const RedPlan p_arg) {
     const RedPlan& p = kernel_plan<S>(p_arg);
     using A = Acc<AK, OP>;
-    __shared__ Store lds[4][64][V];
+    __shared__ typename A::Lds lds[4][64][V];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const uint32_t n_ko = p.n_out / p.lf;
@@ -230,17 +230,17 @@ __global__ __launch_bounds__(256) void reduce_cols_kernel(const RedPlan p_arg) {
             }
         }
 #pragma unroll
-        for (int v = 0; v < V; ++v) lds[wave][lane][v] = acc[v].store();
+        for (int v = 0; v < V; ++v) lds[wave][lane][v] = acc[v].lds();
         __syncthreads();
         if (wave == 0 && active) {
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 if (v < nvalid) {
                     A a;
-                    a.load(lds[0][lane][v]);
+                    a.from_lds(lds[0][lane][v]);
                     for (int w = 1; w < 4; ++w) {
                         A b;
-                        b.load(lds[w][lane][v]);
+                        b.from_lds(lds[w][lane][v]);
                         a.merge(b);
                     }
                     const uint32_t o = ko * p.lf + j0 + v;
@@ -270,9 +270,51 @@ template <class A> __device__ __forceinline__ A wave_group_combine(A a, int G) {
 }
 
 // ---------------------------------------------------------------- split combine
-// p.group (T, a power of two <= 64) lanes per output: lane t merges partials
-// t, t + T, ... in order (loaded 8 at a time), then the T lanes combine in a
-// fixed butterfly, lower lane first
+// A partial as written by another workgroup in the same launch: device-scope
+// loads, which bypass the non-coherent per-XCD L2 (DEV), or plain loads
+// (partials written by an earlier launch).
+template <bool DEV> __device__ __forceinline__ Store load_partial(const Store* p) {
+    if constexpr (DEV) {
+        return Store{__hip_atomic_load(&p->a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                     __hip_atomic_load(&p->b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+    } else {
+        return *p;
+    }
+}
+
+// The split partials of output o merged by T (a power of two <= 64) lanes:
+// lane t merges partials t, t + T, ... in order (loaded 8 at a time), then
+// the T lanes combine in a fixed butterfly, lower lane first.  Every lane of
+// the wave calls it; lanes with o past the outputs contribute the identity.
+template <class A, bool DEV>
+__device__ __forceinline__ A combine_partials(const RedPlan& p, uint32_t o, uint32_t T, uint32_t t) {
+    A a;
+    a.init(p.identity);
+    if (o < p.n_out) {
+        const size_t ps = p.split_major ? size_t(p.n_out) : 1;  // partial stride between splits
+        const Store* part = reinterpret_cast<const Store*>(p.partials) + (p.split_major ? size_t(o) : size_t(o) * p.splits);
+        uint32_t s = t;
+        for (; s + 7 * T < uint32_t(p.splits); s += 8 * T) {
+            Store st[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) st[k] = load_partial<DEV>(part + size_t(s + uint32_t(k) * T) * ps);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                A b;
+                b.load(st[k]);
+                a.merge(b);
+            }
+        }
+        for (; s < uint32_t(p.splits); s += T) {
+            A b;
+            b.load(load_partial<DEV>(part + size_t(s) * ps));
+            a.merge(b);
+        }
+    }
+    return wave_group_combine(a, int(T));
+}
+
+// p.group (T) lanes per output, see combine_partials
 template <int AK, int OP>
 __global__ __launch_bounds__(256) void reduce_combine_kernel(const RedPlan p) {
     using A = Acc<AK, OP>;
@@ -281,30 +323,7 @@ __global__ __launch_bounds__(256) void reduce_combine_kernel(const RedPlan p) {
     const uint32_t t = threadIdx.x & (T - 1);
     for (uint32_t ob = blockIdx.x * per_block; ob < p.n_out; ob += gridDim.x * per_block) {
         const uint32_t o = ob + threadIdx.x / T;  // uniform over each group of T lanes
-        A a;
-        a.init(p.identity);
-        if (o < p.n_out) {
-            const size_t ps = p.split_major ? size_t(p.n_out) : 1;  // partial stride between splits
-            const Store* part = reinterpret_cast<const Store*>(p.partials) + (p.split_major ? size_t(o) : size_t(o) * p.splits);
-            uint32_t s = t;
-            for (; s + 7 * T < uint32_t(p.splits); s += 8 * T) {
-                Store st[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) st[k] = part[size_t(s + uint32_t(k) * T) * ps];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    A b;
-                    b.load(st[k]);
-                    a.merge(b);
-                }
-            }
-            for (; s < uint32_t(p.splits); s += T) {
-                A b;
-                b.load(part[size_t(s) * ps]);
-                a.merge(b);
-            }
-        }
-        a = wave_group_combine(a, int(T));
+        A a = combine_partials<A, false>(p, o, T, t);
         if (o < p.n_out && t == 0) finish(p, a, o);
     }
 }
@@ -449,7 +468,7 @@ template <int V, int NR, class S, int AK, int OP>
 __global__ __launch_bounds__(512) void reduce_cols_fast_kernel(const RedPlan p) {
     using A = Acc<AK, OP>;
     constexpr int U = fast_unroll<NR, V, true>();
-    __shared__ Store lds[512][V];
+    __shared__ typename A::Lds lds[512][V];  // 8-byte pairs: a bf16 tile (V = 8) takes 32 KiB
     const uint32_t TW = uint32_t(p.group);
     const uint32_t RG = 512u / TW;
     const uint32_t tl = threadIdx.x & (TW - 1);
@@ -509,26 +528,59 @@ __global__ __launch_bounds__(512) void reduce_cols_fast_kernel(const RedPlan p) 
     else if (full_tile && ntm == 0) sweep(std::true_type{}, std::integral_constant<int, 0>{});
     else if (active) sweep(std::false_type{}, std::integral_constant<int, 2>{});
 #pragma unroll
-    for (int v = 0; v < V; ++v) lds[threadIdx.x][v] = acc[v].store();
+    for (int v = 0; v < V; ++v) lds[threadIdx.x][v] = acc[v].lds();
     __syncthreads();
+    const bool merge_here = p.splits > 1 && p.counters != nullptr;  // uniform
     if (rg == 0 && active) {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             if (v < nvalid) {
                 A a;
-                a.load(lds[tl][v]);
+                a.from_lds(lds[tl][v]);
                 for (uint32_t g = 1; g < RG; ++g) {
                     A b;
-                    b.load(lds[g * TW + tl][v]);
+                    b.from_lds(lds[g * TW + tl][v]);
                     a.merge(b);
                 }
                 const uint32_t o = j0 + uint32_t(v);
-                if (p.splits == 1) finish(p, a, o);
-                else if (p.split_major) reinterpret_cast<Store*>(p.partials)[size_t(sp) * p.n_out + o] = a.store();
-                else reinterpret_cast<Store*>(p.partials)[size_t(o) * p.splits + sp] = a.store();
+                if (p.splits == 1) {
+                    finish(p, a, o);
+                } else if (merge_here) {  // split-major, written through to memory
+                    const Store st = a.store();
+                    Store* dst = reinterpret_cast<Store*>(p.partials) + size_t(sp) * p.n_out + o;
+                    __hip_atomic_store(&dst->a, st.a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&dst->b, st.b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                } else if (p.split_major) {
+                    reinterpret_cast<Store*>(p.partials)[size_t(sp) * p.n_out + o] = a.store();
+                } else {
+                    reinterpret_cast<Store*>(p.partials)[size_t(o) * p.splits + sp] = a.store();
+                }
             }
         }
     }
+    if (!merge_here) return;
+    // the last of the tile's splits to arrive merges its partials: every
+    // thread's partial stores complete, then one arrival count per block
+    __shared__ uint32_t s_last;
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    if (threadIdx.x == 0)
+        s_last = __hip_atomic_fetch_add(p.counters + tile, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                 uint32_t(p.splits) - 1u;
+    __syncthreads();
+    if (!s_last) return;  // uniform in the block
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const uint32_t T = uint32_t(p.combine_group);
+    const uint32_t per_pass = 512u / T;
+    const uint32_t o0 = tile * TW * uint32_t(V);  // the tile's first output
+    const uint32_t o1 = min(p.n_out, o0 + TW * uint32_t(V));
+    for (uint32_t ob = o0; ob < o1; ob += per_pass) {
+        const uint32_t o = ob + threadIdx.x / T;  // uniform over each group of T lanes
+        A a = combine_partials<A, true>(p, o < o1 ? o : p.n_out, T, threadIdx.x & (T - 1));
+        if (o < o1 && (threadIdx.x & (T - 1)) == 0) finish(p, a, o);
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(p.counters + tile, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace xg

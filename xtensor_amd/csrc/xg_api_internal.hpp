@@ -32,6 +32,7 @@ struct Arena {
     char* base = nullptr;
     size_t cap = 0;
     size_t used = 0;
+    uint32_t* counters = nullptr;  // column reductions' arrival counters, or none
     // `bytes` (256-byte aligned) from the block; nullptr for 0 bytes
     char* take(size_t bytes);
 };
@@ -41,9 +42,14 @@ struct Arena {
 using BlockSource = char* (*)(size_t bytes, hipStream_t stream, void* ctx);
 
 // out = reduce(e) (every reduction kind of xg_reduce), with its temporaries
-// and split partials in one block from `src`.
+// and split partials in one block from `src`; `counters`: the column
+// reductions may merge their splits themselves with the stream's arrival
+// counters (allocated on first use), else a combine kernel does.
 void reduce_call(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, hipStream_t stream,
-                 BlockSource src, void* ctx);
+                 BlockSource src, void* ctx, bool counters);
+
+// the stream's arrival counters (zeroed, kReduceCounters words)
+uint32_t* reduce_counters(hipStream_t stream);
 
 // Plan out = e with the compiled-kernel vector width, falling back to the
 // generic interpreter's width when no compiled program matches.
