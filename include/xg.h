@@ -329,6 +329,16 @@ xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out,
 xg_status xg_assign_indexed(const xg_expr* values, const xg_expr* offsets, const xg_tensor* target,
                             xg_stream_t stream);
 
+/* xt::argwhere<L>(cond) (core/xoperation.hpp:893-930; the indices of
+ * xt::filter, views/xindex_view.hpp:820-826): out (int64 [count, ndim], any
+ * strides) receives the multi-index of every element of cond that converts
+ * to true, in row-major (column_major = 0) or column-major traversal order.
+ * count, the number of such elements, is the caller's (for example from
+ * xg_reduce of sum(cast<int64>(cast<bool>(cond)))); rows past out's first
+ * dimension are not written.  Stream-ordered: a scan of the flags on the
+ * device, no host copy of the condition. */
+xg_status xg_argwhere(const xg_expr* cond, int32_t column_major, const xg_tensor* out, xg_stream_t stream);
+
 /* ---- .npy files to and from device memory (io/xnpy.hpp) ------------------
  * Header of an .npy file: dtype, shape and order (xt::detail::load_npy_file's
  * header parse, io/xnpy.hpp:236-393, 646-673).  A type string without a

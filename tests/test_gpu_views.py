@@ -202,3 +202,60 @@ def test_gather_with_runtime_compiled_kernel():
         A.lib().xg_debug_force_dynamic(0)
     assert kern.startswith("jit:")
     assert same_bits(host(out), host(out2))
+
+
+# ---------------------------------------------------------------- argwhere on the device
+def test_kat_argwhere():
+    """test_xoperation.cpp:562-595: argwhere of a 1-d int array, a 2-d int
+    array, equal(b, 0), and an all-true [3, 3, 3] bool array."""
+    a = xt.from_numpy(np.array([1, 0, 3], np.int32))
+    assert np.array_equal(host(xt.argwhere(a)), [[0], [2]])
+    assert xt.last_kernel() == "argwhere"
+    b = xt.from_numpy(np.array([[0, 2, 1], [2, 1, 0]], np.int32))
+    assert np.array_equal(host(xt.argwhere(b)), [[0, 1], [0, 2], [1, 0], [1, 1]])
+    assert np.array_equal(host(xt.argwhere(xt.equal(b, 0))), [[0, 0], [1, 2]])
+    d = xt.from_numpy(np.ones((3, 3, 3), bool))
+    dn = host(xt.argwhere(d))
+    assert dn.shape == (27, 3) and np.array_equal(dn[-1], [2, 2, 2])
+    assert xt.count_nonzero(d) == 27
+
+
+@pytest.mark.parametrize("layout", ["row_major", "column_major"])
+def test_argwhere_against_numpy(layout):
+    """Random conditions against numpy's argwhere (column-major order: the
+    argwhere of the transpose with the index components reversed): a float
+    comparison, an integer array read as truth values, a 3-d comparison with
+    a broadcast operand, an empty result, and a [8192, 4096] float32 case."""
+    rng = np.random.default_rng(51)
+
+    def ref(c):
+        c = np.asarray(c, bool)
+        if layout == "column_major":
+            return np.argwhere(c.T)[:, ::-1]
+        return np.argwhere(c)
+
+    x = rng.uniform(0, 1, (1000, 300)).astype(np.float32)
+    X = xt.from_numpy(x)
+    assert np.array_equal(host(xt.argwhere(X > 0.5, layout)), ref(x > 0.5))
+    k = (rng.integers(0, 3, (77, 33))).astype(np.int64)
+    assert np.array_equal(host(xt.argwhere(xt.from_numpy(k), layout)), ref(k != 0))
+    a3 = rng.normal(0, 1, (20, 30, 40))
+    b1 = rng.normal(0, 1, (40,))
+    A3, B1 = xt.from_numpy(a3), xt.from_numpy(b1)
+    assert np.array_equal(host(xt.argwhere(A3 > B1, layout)), ref(a3 > b1))
+    got = xt.argwhere(A3 > 100.0, layout)
+    assert got.shape == (0, 3)
+    big = rng.normal(1.0, 1.0, (8192, 4096)).astype(np.float32)
+    BIG = xt.from_numpy(big)
+    assert np.array_equal(host(xt.argwhere(BIG > 1.5, layout)), ref(big > 1.5))
+
+
+def test_filter_reads_device_indices():
+    """filter(e, cond) = index_view(e, argwhere(cond)) with the indices
+    listed on the device: values and their order match numpy's boolean
+    selection (row-major), also for a large array."""
+    rng = np.random.default_rng(52)
+    e = rng.normal(0, 1, (4096, 1000)).astype(np.float32)
+    E = xt.from_numpy(e)
+    got = host(xt.eval(xt.filter(E, E > 1.0)))
+    assert np.array_equal(got, e[e > 1.0])

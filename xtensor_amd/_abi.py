@@ -154,6 +154,7 @@ EXPORTS = {
     "xg_scan_info": (C.c_int, [_P(Expr), _P(ScanDesc), _P(C.c_int32), _P(C.c_int32), _P(C.c_int64)]),
     "xg_scan": (C.c_int, [_P(Expr), _P(ScanDesc), _P(Tensor), C.c_void_p]),
     "xg_assign_indexed": (C.c_int, [_P(Expr), _P(Expr), _P(Tensor), C.c_void_p]),
+    "xg_argwhere": (C.c_int, [_P(Expr), C.c_int32, _P(Tensor), C.c_void_p]),
     "xg_npy_info": (C.c_int, [C.c_char_p, _P(C.c_int32), _P(C.c_int32), _P(C.c_int64), _P(C.c_int32)]),
     "xg_npy_load": (C.c_int, [C.c_char_p, _P(Tensor), C.c_void_p]),
     "xg_npy_dump": (C.c_int, [C.c_char_p, _P(Tensor), C.c_void_p]),

@@ -891,14 +891,31 @@ def index_view(e, indices) -> IndexView:
     return IndexView(e, indices)
 
 
-def argwhere(cond, layout="row_major") -> np.ndarray:
-    """xt::argwhere<L>: multi-indices of the true elements in row-major (or
-    column-major) traversal order (the condition is evaluated on the device,
-    the positions listed on the host)."""
-    c = eval(cast(_wrap(cond), "bool")).to_numpy()
-    if layout == "column_major":
-        return np.argwhere(c.T)[:, ::-1].copy()
-    return np.argwhere(c)
+def count_nonzero(e) -> int:
+    """xt::count_nonzero(e) (core/xmath.hpp:2495-2505): the number of
+    elements that convert to true (a device reduction; the count is read
+    back)."""
+    return int(eval(sum(cast(cast(_wrap(e), "bool"), "int64"))).to_numpy())
+
+
+def argwhere(cond, layout="row_major") -> DeviceArray:
+    """xt::argwhere<L> (core/xoperation.hpp:893-930): int64 [count, ndim]
+    multi-indices of the true elements in row-major (or column-major)
+    traversal order, listed on the device (xg_argwhere: a scan of the flags
+    and one compaction kernel); only the count is read back, to size the
+    result."""
+    c = _wrap(cond)
+    if layout not in ("row_major", "column_major"):
+        raise ValueError("layout is 'row_major' or 'column_major'")
+    _, shp = expr_info(c)
+    n = count_nonzero(c)
+    out = empty((n, len(shp)), "int64")
+    if n:
+        _require_device(_leaves(c) + [out])
+        s = serialize(c, reducer_fn=_eval_reducer)
+        t = out.to_abi()
+        _check(A.lib().xg_argwhere(s.ptr(), 1 if layout == "column_major" else 0, C.byref(t), _stream()))
+    return out
 
 
 def filter(e, cond, layout="row_major") -> IndexView:  # noqa: A001
