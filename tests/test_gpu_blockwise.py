@@ -63,3 +63,44 @@ def test_memmap_float_stream(tmp_path):
     np.testing.assert_allclose(m1.to_numpy(), h64.mean(1), rtol=2e-7)
     np.testing.assert_allclose(v.to_numpy()[()], h64.var(ddof=1), rtol=1e-10)
     assert mx.to_numpy()[()] == h.max()
+
+
+def test_against_the_oracle():
+    """Blockwise reductions against the oracle's restatement of the whole
+    reduction (reducers/xreducer.hpp fold order), not against the library's
+    own single pass: integers and extrema exactly (int32 KAT input, six axis
+    sets), float32 sums / means / norms of a host array streamed in chunks
+    within 2e-7 of the oracle's long-double reference."""
+    from helpers import oracle_for, Dual
+    ints = Dual.of(kat_input())
+    oi = oracle_for(ints)
+    for kind in ("sum", "prod", "amax", "amin", "norm_l1", "norm_linf", "norm_sq"):
+        for axes in AXES:
+            got = getattr(xt.blockwise, kind)(ints.host, (5, 4, 2), axes).to_numpy()
+            ref = oi.reduce(xt.Reducer(kind, ints.dev, axes))
+            assert got.dtype == ref.dtype and np.array_equal(got, ref), (kind, axes)
+    rng = np.random.default_rng(7)
+    f = Dual.of(rng.normal(1.0, 1.0, (600, 500)).astype(np.float32))
+    of = oracle_for(f)
+    for kind in ("sum", "mean", "norm_l2", "amax", "amin"):
+        for axes in ([0], [1], None):
+            got = getattr(xt.blockwise, kind)(f.host, (128, 96), axes).to_numpy()
+            r = xt.Reducer(kind, f.dev, axes)
+            if kind in ("amax", "amin"):
+                assert np.array_equal(got, of.reduce(r)), (kind, axes)
+            else:
+                ref = of.reduce(r, high_precision=True)
+                err = np.max(np.abs(got.astype(np.float64) - ref) / np.abs(ref))
+                assert err <= 2e-7, (kind, axes, err)
+
+
+def test_stream_overlap_many_chunks():
+    """A host array in many chunks (the two upload buffers are reused many
+    times while reductions are in flight): every chunk's data arrives intact
+    (exact integer sums against numpy)."""
+    rng = np.random.default_rng(8)
+    h = rng.integers(-1000, 1000, (4096, 777)).astype(np.int32)
+    got = xt.blockwise.sum(h, (100, 100), [0], dtype="int64").to_numpy()
+    assert np.array_equal(got, h.astype(np.int64).sum(0))
+    got1 = xt.blockwise.sum(h, (97, 777), [1], dtype="int64").to_numpy()
+    assert np.array_equal(got1, h.astype(np.int64).sum(1))

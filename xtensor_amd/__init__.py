@@ -21,6 +21,7 @@ from .core import (  # noqa: F401
 )
 
 from . import blockwise  # noqa: F401,E402
+from .chunked import ChunkedArray, chunked_array, is_chunked  # noqa: F401,E402
 from .batch import Batch  # noqa: F401,E402
 from .graph import PlanGraph  # noqa: F401,E402
 from .core import execute_many  # noqa: F401,E402

@@ -4,7 +4,8 @@ and merge the chunk results, so an input larger than device memory — a host
 array or a numpy memmap — streams through the GPU one chunk at a time.
 
 `xt.blockwise.sum(e, chunk_shape, axes)` etc.: every chunk is reduced on the
-device with kept dims, and its result merged into the slice of the output
+device with kept dims (host chunks are uploaded one ahead of the
+computation, on a stream of their own, through pinned buffers), and its result merged into the slice of the output
 its kept coordinates cover — `+=` for sums, `*=` for products, max / min for
 extrema (the merges of xblockwise_reducer_functors.hpp).  Float sums merge
 in float64 and round once at the end; integer merges wrap exactly like the
@@ -22,6 +23,7 @@ import itertools
 import numpy as np
 
 from . import core as _c
+from .chunked import ChunkedArray
 
 _MERGE = {"sum": "add", "prod": "mul", "amax": "max", "amin": "min", "mean": "add", "norm_l0": "add",
           "norm_l1": "add", "norm_sq": "add", "norm_l2": "add", "norm_linf": "max"}
@@ -38,14 +40,84 @@ def _norm_axes(axes, nd):
     return ax
 
 
-def _source(e):
-    """(shape, dtype code, chunk getter) of a host array or a device array."""
+class _HostUpload:
+    """Host chunks onto the device one chunk ahead of the computation: two
+    pinned staging buffers and two device buffers, the copies on their own
+    stream.  Chunk k+1 is gathered into pinned memory and its upload enqueued
+    before chunk k is handed to the caller, whose work on the current stream
+    waits only for chunk k's upload; a buffer is reused once the work that
+    read it (recorded after the caller's turn) has finished."""
+
+    def __init__(self, get, code, chunk_shape):
+        import torch
+        self.get = get  # slices -> the host chunk
+        self.code = code
+        itemsize = _c.A.DTYPE_SIZE[code]
+        nbytes = max(1, int(np.prod(chunk_shape, dtype=np.int64)) * itemsize)
+        self.pinned = [torch.empty(nbytes, dtype=torch.uint8, pin_memory=True) for _ in range(2)]
+        self.dev = [torch.empty(nbytes, dtype=torch.uint8, device="cuda") for _ in range(2)]
+        self.copy = torch.cuda.Stream()
+        self.ready = [torch.cuda.Event() for _ in range(2)]  # upload into slot done
+        self.freed = [torch.cuda.Event() for _ in range(2)]  # work reading slot done
+        self.used = [False, False]
+
+    def _upload(self, k, sl):
+        import torch
+        slot = k % 2
+        if self.used[slot]:
+            self.ready[slot].synchronize()  # the pinned buffer's previous upload has left it
+        arr = np.ascontiguousarray(self.get(sl))
+        n = arr.nbytes
+        np.copyto(self.pinned[slot][:n].numpy(), arr.reshape(-1).view(np.uint8))
+        with torch.cuda.stream(self.copy):
+            if self.used[slot]:
+                self.copy.wait_event(self.freed[slot])
+            self.dev[slot][:n].copy_(self.pinned[slot][:n], non_blocking=True)
+            self.ready[slot].record(self.copy)
+        self.used[slot] = True
+        return _c.DeviceArray(self.dev[slot], self.code, arr.shape)
+
+    def chunks(self, slices):
+        import torch
+        slices = list(slices)
+        if not slices:
+            return
+        compute = torch.cuda.current_stream()
+        nxt = self._upload(0, slices[0])
+        for k, sl in enumerate(slices):
+            cur = nxt
+            if k + 1 < len(slices):
+                nxt = self._upload(k + 1, slices[k + 1])
+            compute.wait_event(self.ready[k % 2])
+            yield sl, cur
+            self.freed[k % 2].record(compute)  # the caller's work on this chunk is enqueued
+
+
+def _source(e, chunk_shape):
+    """(shape, dtype code, iterator factory) of a host array or host chunked
+    array (chunks uploaded one ahead of the computation), a device array
+    (chunks are views) or a device chunked array (its chunks)."""
+    if isinstance(e, ChunkedArray):
+        if tuple(chunk_shape) != e.chunk_shape:
+            raise ValueError("a chunked array is reduced with its own chunk shape")
+
+        def grid_index(sl):
+            return tuple(s.start // c for s, c in zip(sl, e.chunk_shape))
+        if e.storage == "device":
+            def it(slices):
+                for sl in slices:
+                    yield sl, e.chunk(*grid_index(sl))
+            return e.shape, e.dtype_code, it
+        return e.shape, e.dtype_code, lambda slices: _HostUpload(
+            lambda sl: e.chunk(*grid_index(sl)), e.dtype_code, e.chunk_shape).chunks(slices)
     if isinstance(e, _c.DeviceArray):
-        def get(sl):
-            return _c.view(e, *[_c.range_(s.start, s.stop) for s in sl])
-        return tuple(e.shape), e.dtype_code, get
+        def it(slices):
+            for sl in slices:
+                yield sl, _c.view(e, *[_c.range_(s.start, s.stop) for s in sl])
+        return tuple(e.shape), e.dtype_code, it
     h = np.asarray(e) if not isinstance(e, np.memmap) else e
-    return tuple(h.shape), _c.dtype_code(h.dtype), lambda sl: _c.from_numpy(np.ascontiguousarray(h[sl]))
+    code = _c.dtype_code(h.dtype)
+    return tuple(h.shape), code, lambda slices: _HostUpload(lambda sl: h[sl], code, tuple(chunk_shape)).chunks(slices)
 
 
 def _chunks(shape, chunk_shape):
@@ -54,11 +126,23 @@ def _chunks(shape, chunk_shape):
         yield tuple(slice(s, min(s + c, n)) for s, c, n in zip(start, chunk_shape, shape))
 
 
+def _whole(e, shape):
+    """The whole source as one device array (for empty sources)."""
+    if isinstance(e, _c.DeviceArray):
+        return e
+    if isinstance(e, ChunkedArray):
+        return _c.from_numpy(e.to_numpy())
+    return _c.from_numpy(np.ascontiguousarray(np.asarray(e)))
+
+
 def _reduce(kind, e, chunk_shape, axes=None, keep_dims=False, dtype=None, ddof=0):
-    shape, code, get = _source(e)
-    nd = len(shape)
+    if chunk_shape is None and isinstance(e, ChunkedArray):
+        chunk_shape = e.chunk_shape
+    nd = len(e.shape) if hasattr(e, "shape") else np.ndim(e)
     if chunk_shape is None or len(chunk_shape) != nd or any(int(c) <= 0 for c in chunk_shape):
         raise ValueError("chunk_shape needs one positive extent per dimension")
+    chunk_shape = tuple(int(c) for c in chunk_shape)
+    shape, code, chunks_of = _source(e, chunk_shape)
     ax = _norm_axes(axes, nd)
     ph = _c._Placeholder(code, shape)
     if kind in ("variance", "stddev"):
@@ -74,8 +158,8 @@ def _reduce(kind, e, chunk_shape, axes=None, keep_dims=False, dtype=None, ddof=0
     kshape = tuple(1 if k in ax else n for k, n in enumerate(shape))  # kept-dims result shape
     acc = _c.empty(kshape, acc_dt)
     seen = np.zeros([(-(-n // c)) if k not in ax else 1 for k, (n, c) in enumerate(zip(shape, chunk_shape))], bool)
-    for sl in _chunks(shape, chunk_shape):
-        part = _c.eval(_c.Reducer(pkind, get(sl), ax, True, None, dtype))
+    for sl, chunk in chunks_of(_chunks(shape, chunk_shape)):
+        part = _c.eval(_c.Reducer(pkind, chunk, ax, True, None, dtype))
         osl = tuple(slice(0, 1) if k in ax else s for k, s in enumerate(sl))
         dst = _c.view(acc, *[_c.range_(s.start, s.stop) for s in osl])
         gi = tuple(0 if k in ax else s.start // chunk_shape[k] for k, s in enumerate(sl))
@@ -92,7 +176,7 @@ def _reduce(kind, e, chunk_shape, axes=None, keep_dims=False, dtype=None, ddof=0
         else:
             _c.assign(dst, _c.minimum(dst, part))
     if 0 in shape:  # no chunks: the single-pass reducer gives the identity / NaN mean
-        return _c.eval(_c.Reducer(kind, get(tuple(slice(0, n) for n in shape)), ax, keep_dims, None, dtype))
+        return _c.eval(_c.Reducer(kind, _whole(e, shape), ax, keep_dims, None, dtype))
     if kind == "mean":
         # mean_division: the sum divided by T-or-double(size / outputs)
         n_red = int(np.prod([shape[k] for k in ax], dtype=np.int64)) if ax else 1
@@ -108,16 +192,16 @@ def _reduce(kind, e, chunk_shape, axes=None, keep_dims=False, dtype=None, ddof=0
 def _moment(kind, e, chunk_shape, ax, keep_dims, dtype, ddof, ph):
     if dtype is not None and _c.dtype_name(_c.dtype_code(dtype)) not in ("float32", "float64"):
         raise NotImplementedError("blockwise variance / stddev with an integral accumulator type")
-    shape, code, get = _source(e)
+    shape, code, chunks_of = _source(e, chunk_shape)
     rdt, rshape = _c.reduce_info(_c.Reducer(kind, ph, ax, keep_dims, None, dtype, ddof))
     m = _reduce("mean", e, chunk_shape, ax, True, dtype)          # kept dims
     # sum of squared deviations, merged in float64
     kshape = tuple(1 if k in ax else n for k, n in enumerate(shape))
     acc = _c.zeros(kshape, "float64")
-    for sl in _chunks(shape, chunk_shape):
+    for sl, chunk in chunks_of(_chunks(shape, chunk_shape)):
         osl = [_c.range_(0, 1) if k in ax else _c.range_(s.start, s.stop) for k, s in enumerate(sl)]
         mv = _c.view(m, *osl)
-        d = get(sl) - mv
+        d = chunk - mv
         part = _c.eval(_c.Reducer("sum", d * d, ax, True, None, None))
         dst = _c.view(acc, *osl)
         _c.assign(dst, dst + part)
