@@ -259,3 +259,20 @@ def test_filter_reads_device_indices():
     E = xt.from_numpy(e)
     got = host(xt.eval(xt.filter(E, E > 1.0)))
     assert np.array_equal(got, e[e > 1.0])
+
+
+def test_functor_view_real_imag():
+    """xfunctor_view (views/xfunctor_view.hpp:400-470) as a node of the fused
+    expression; real(e) of real data is e itself and assignable; imag(e) is
+    zeros of e's type, NaN included."""
+    rng = np.random.default_rng(53)
+    x = rng.normal(0, 1, (300, 200))
+    x[0, 0] = np.nan
+    X = xt.from_numpy(x)
+    fv = xt.eval(xt.functor_view(X, lambda v: xt.abs(v))).to_numpy()
+    assert np.array_equal(fv, np.abs(x), equal_nan=True)
+    assert np.array_equal(xt.eval(xt.real(X)).to_numpy(), x, equal_nan=True)
+    im = xt.eval(xt.imag(X)).to_numpy()
+    assert im.dtype == np.float64 and np.array_equal(im, np.zeros_like(x))
+    xt.noalias(xt.real(X)).assign(X * 2.0)
+    assert np.array_equal(X.to_numpy(), x * 2.0, equal_nan=True)

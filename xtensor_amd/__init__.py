@@ -10,7 +10,7 @@ ABI of `include/xg.h`; there is no CPU fallback.
 from .core import (  # noqa: F401
     BroadcastError, DeviceArray, DeviceError, Expr, Func, Reducer, Scalar, Scan, assign_checked,
     Filtration, IndexView, abs, acos, all_, amax, amin, argwhere, asarray, asin, assign, atan, atan2, broadcast,
-    cast, cbrt, ceil, clip, count_nonzero, filter, filtration, index_view, masked_view,
+    cast, cbrt, ceil, clip, count_nonzero, filter, filtration, functor_view, imag, index_view, masked_view, real,
     cos, cosh, cumprod, cumsum, dtype_code, dtype_name, dump_npy, empty, equal, erf, eval, exp, exp2, expm1,
     expr_info, fabs, floor, fmod, from_numpy, greater, hypot, last_kernel, less, log, log1p, log2,
     load_npy, log10, logical_and, logical_not, logical_or, maximum, mean, minimum, minmax, newaxis, noalias,

@@ -891,6 +891,31 @@ def index_view(e, indices) -> IndexView:
     return IndexView(e, indices)
 
 
+def functor_view(e, f):
+    """xt::xfunctor_view<F, E> (views/xfunctor_view.hpp:400-470): e's elements
+    seen through the functor f, applied lazily when the view is read; f maps
+    an expression to an expression (f = lambda x: xt.abs(x), ...), so the
+    view is one more node of the fused expression.  xfunctor_view is an
+    lvalue only for functors that return a reference into the element (the
+    real / imaginary parts of complex values, views/xcomplex.hpp); with the
+    real element types here, real(e) is that view and is assignable."""
+    return f(_wrap(e))
+
+
+def real(e):
+    """xt::real(e) of a real-valued expression: e itself (an lvalue view of a
+    container, as xt::real's functor view is)."""
+    return e if isinstance(e, DeviceArray) else _wrap(e)
+
+
+def imag(e):
+    """xt::imag(e) of a real-valued expression: zeros of e's shape and value
+    type (read-only), also where e is NaN."""
+    e = _wrap(e)
+    dt, _ = expr_info(e)
+    return where(scalar(True, "bool"), scalar(0, dtype_name(dt)), e)
+
+
 def count_nonzero(e) -> int:
     """xt::count_nonzero(e) (core/xmath.hpp:2495-2505): the number of
     elements that convert to true (a device reduction; the count is read
