@@ -287,6 +287,23 @@ def uneven_batch(np, torch, xt, steps=20, n_inst=4096):
             p.run()
     torch.cuda.synchronize()
     plain = (time.perf_counter() - t0) / reps
+    # the same plans enqueued from C: one xg_plan_execute per plan in a C loop
+    plain_c = None
+    loop_lib = os.path.join(os.path.dirname(os.path.abspath(__file__)), "xtensor_amd", "lib", "libxgloop.so")
+    if os.path.exists(loop_lib):
+        import ctypes
+
+        ll = ctypes.CDLL(loop_lib)
+        ll.xgl_time_plans.restype = ctypes.c_int32
+        ll.xgl_time_plans.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p,
+                                      ctypes.POINTER(ctypes.c_double)]
+        handles = (ctypes.c_void_p * len(plans))(*[p.handle.value for p in plans])
+        secs = ctypes.c_double()
+        st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        rc = ll.xgl_time_plans(handles, len(plans), reps, st, ctypes.byref(secs))
+        if rc != 0:
+            raise RuntimeError(f"xgl_time_plans failed with status {rc}")
+        plain_c = secs.value / reps
     nbytes = 8 * elems
     return {
         "instances": n_inst, "elements": elems, "algorithmic_bytes": nbytes,
@@ -294,6 +311,9 @@ def uneven_batch(np, torch, xt, steps=20, n_inst=4096):
         "batched_inst_per_s": round(n_inst / wall, 1), "batched_GBps": round(nbytes / (kms * 1e-3) / 1e9, 1),
         "plain_ms": round(plain * 1e3, 3), "plain_inst_per_s": round(n_inst / plain, 1),
         "plain_launches": len(plans), "speedup_vs_plain": round(plain / wall, 1),
+        "plain_from_c_ms": None if plain_c is None else round(plain_c * 1e3, 3),
+        "plain_from_c_inst_per_s": None if plain_c is None else round(n_inst / plain_c, 1),
+        "speedup_vs_plain_from_c": None if plain_c is None else round(plain_c / wall, 1),
         "max_rel_err_vs_fp64": worst, "tuning": tuning, "tasks": b.info()["tasks"],
     }
 
@@ -475,22 +495,56 @@ def serve_requests(np, n_req, seed=4):
     return out
 
 
-def serving(np, torch, xt, rates=(1000, 10000, 50000), n_req=300, seed=4):
+def serving(np, torch, xt, rates=(1000, 10000, 50000), n_req=3000, seed=4):
     """Served request stream: Poisson arrivals at each rate (seed 4), operands
     resident in HBM, 64 GB admission budget; continuous batching (4 streams,
     up to 64 requests per launch) vs the plain per-request path (one stream,
-    each request alone through the per-expression calls)."""
+    each request alone through the per-expression calls).
+
+    Every request reads its own slice of a resident input pool and writes its
+    own slice of an output pool: consecutive requests of one (template, dtype)
+    take consecutive regions, the pools are several GB (far past the 256 MB
+    MALL), so a request's operands are not cache-resident from an earlier
+    one.  Request descriptors are built before the timed stream, so the
+    submission loop is the arrival clock and one library call per request."""
     from xtensor_amd.serve import Server, now_ns
 
     reqs = serve_requests(np, n_req, seed)
     esz = {"float32": 4, "float64": 8, "int32": 4}
-    # one resident input pool per (template, dtype), requests read a prefix of it
-    maxn = {}
+    align = 64  # elements: slices start 256-byte aligned
+    rnd = lambda n: (n + align - 1) // align * align  # noqa: E731
+    # input pools per (template, dtype): requests take consecutive regions,
+    # wrapping at 4x the largest request (at least 1 GB)
+    maxn, pool_n = {}, {}
     for tpl, dt, shape, _ in reqs:
-        maxn[(tpl, dt)] = max(maxn.get((tpl, dt), 0), int(np.prod(shape)))
+        maxn[(tpl, dt)] = max(maxn.get((tpl, dt), 0), rnd(int(np.prod(shape))))
+    for key, n in maxn.items():
+        pool_n[key] = max(4 * n, (1 << 30) // esz[key[1]])
+    in_off, cur = [], {}
+    for tpl, dt, shape, _ in reqs:
+        n, c = rnd(int(np.prod(shape))), cur.get((tpl, dt), 0)
+        if c + n > pool_n[(tpl, dt)]:
+            c = 0
+        in_off.append(c)
+        cur[(tpl, dt)] = c + n
+    # output pools per dtype, regions taken in request order, at most 16 GiB
+    # each: two requests share output memory only with 16 GiB of that
+    # dtype's outputs between them (peak_inflight_GB, inputs and outputs
+    # together, says how much is in flight at once)
+    out_total = {}
+    for _, dt, _, oshape in reqs:
+        out_total[dt] = out_total.get(dt, 0) + rnd(int(np.prod(oshape)))
+    out_pool_n = {dt: min(n, (16 << 30) // esz[dt]) for dt, n in out_total.items()}
+    out_off, ocur, wraps = [], {}, 0
+    for _, dt, _, oshape in reqs:
+        n, c = rnd(int(np.prod(oshape))), ocur.get(dt, 0)
+        if c + n > out_pool_n[dt]:
+            c, wraps = 0, wraps + 1
+        out_off.append(c)
+        ocur[dt] = c + n
     gen = torch.Generator(device="cuda").manual_seed(seed)
     pools, params = {}, {}
-    for (tpl, dt), n in maxn.items():
+    for (tpl, dt), n in pool_n.items():
         if dt == "int32":
             t = torch.randint(-2**20, 2**20, (n,), dtype=torch.int32, device="cuda", generator=gen)
         else:
@@ -500,34 +554,39 @@ def serving(np, torch, xt, rates=(1000, 10000, 50000), n_req=300, seed=4):
             pd = getattr(torch, dt)
             params[dt] = [torch.rand(16, dtype=pd, device="cuda", generator=gen) * s + o
                           for s, o in ((0.2, -0.1), (1.5, 0.5), (1.0, 0.5), (0.2, -0.1))]
-    nbytes = []
-    for tpl, dt, shape, oshape in reqs:
-        nbytes.append((int(np.prod(shape)) + int(np.prod(oshape))) * esz[dt])
+    opools = {dt: torch.empty(n, dtype=getattr(torch, dt), device="cuda") for dt, n in sorted(out_pool_n.items())}
+    nbytes = [(int(np.prod(shape)) + int(np.prod(oshape))) * esz[dt] for _, dt, shape, oshape in reqs]
 
     def view(t, shape):
         return xt.DeviceArray(t.view(torch.uint8), str(t.dtype).replace("torch.", ""), shape)
 
-    result = {"requests_per_rate": n_req, "budget_GB": 64, "mean_request_MB": round(float(np.mean(nbytes)) / 1e6, 2)}
+    pviews = {dt: [view(p, (16,)) for p in ps] for dt, ps in params.items()}
+    prepared = []
+    for k, (tpl, dt, shape, oshape) in enumerate(reqs):
+        x = view(pools[(tpl, dt)][in_off[k]:in_off[k] + int(np.prod(shape))], shape)
+        o = view(opools[dt][out_off[k]:out_off[k] + int(np.prod(oshape))], oshape)
+        prepared.append(Server.prepare(tpl, x, o, pviews[dt] if tpl == "affine_clip" else [], tag=k))
+    pool_GB = (sum(t.numel() * t.element_size() for t in pools.values()) +
+               sum(t.numel() * t.element_size() for t in opools.values())) / 1e9
+    result = {"requests_per_rate": n_req, "budget_GB": 64, "mean_request_MB": round(float(np.mean(nbytes)) / 1e6, 2),
+              "operands": "distinct slices per request", "pool_GB": round(pool_GB, 1),
+              "output_pool_wraps": wraps}
 
     def stream(mode, gaps):
-        outs = [torch.empty(int(np.prod(o)), dtype=getattr(torch, dt), device="cuda") for _, dt, _, o in reqs]
         cfg = dict(budget_bytes=64 << 30, queue_capacity=1 << 20)
         cfg.update(dict(streams=4, max_batch=64) if mode == "batched" else dict(streams=1, max_batch=1, plain=True))
         torch.cuda.synchronize()
         with Server(**cfg) as srv:
             t0 = now_ns() + 2_000_000
             arr = t0 + np.cumsum(gaps * 1e9).astype(np.int64)
-            for k, (tpl, dt, shape, oshape) in enumerate(reqs):
-                while now_ns() < arr[k]:
+            arr_l = arr.tolist()
+            for k in range(n_req):
+                while now_ns() < arr_l[k]:
                     pass
-                x = view(pools[(tpl, dt)][:int(np.prod(shape))], shape)
-                o = view(outs[k], oshape)
-                ps = [view(p, (16,)) for p in params[dt]] if tpl == "affine_clip" else []
-                srv.submit(tpl, x, o, ps, tag=k, arrival_ns=int(arr[k]))
+                srv.submit_prepared(prepared[k], arr_l[k])
             srv.drain(600_000)
             comps = srv.poll(n_req + 16)
             st = srv.stats()
-        del outs
         return arr, comps, st
 
     # one untimed pass of the whole request list per mode first: the first
@@ -545,9 +604,11 @@ def serving(np, torch, xt, rates=(1000, 10000, 50000), n_req=300, seed=4):
                 "req_per_s": round(n_req / span, 1), "GBps": round(sum(nbytes) / span / 1e9, 1),
                 "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                 "launches": st["launches"], "peak_inflight_GB": round(st["peak_inflight_bytes"] / 1e9, 2),
+                "arrival_window_ms": round((int(arr[-1]) - int(arr[0])) / 1e6, 2),
                 "ok": int(sum(c["status"] == 0 for c in comps)),
             }
             print(f"served stream {mode} at {rate} req/s: {result[f'{mode}_{rate}']}", file=sys.stderr, flush=True)
+    del pools, opools
     return result
 
 

@@ -124,6 +124,15 @@ def build(force: bool = False, jobs: int | None = None) -> str:
               "-I", os.path.join(HERE, "cpp"), cpp_test, "-o", exe, "-L", LIBDIR, "-lxg",
               "-L", "/opt/rocm/lib", "-lamdhip64", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN/../lib",
               "-Wl,-rpath,/opt/rocm/lib"])
+
+    # 4. the C plan-loop timer (a C caller of xg_plan_execute, used by the bench)
+    loop_src = os.path.join(HERE, "cpp", "plan_loop.cpp")
+    loop_lib = os.path.join(LIBDIR, "libxgloop.so")
+    if os.path.exists(loop_src) and (force or not os.path.exists(loop_lib) or
+                                     _newest([loop_src, LIB] + headers) > os.path.getmtime(loop_lib)):
+        _run(["g++", "-std=c++17", "-O2", "-shared", "-fPIC", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include",
+              "-I", INCLUDE, loop_src, "-o", loop_lib, "-L", LIBDIR, "-lxg", "-L", "/opt/rocm/lib", "-lamdhip64",
+              "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"])
     return LIB
 
 

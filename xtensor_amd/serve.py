@@ -55,6 +55,24 @@ class Server:
     def submit(self, template: str, x: DeviceArray, out: DeviceArray, params: Sequence[DeviceArray] = (),
                tag: int = 0, arrival_ns: int = 0) -> int:
         """Queue a request; returns the library status (0 = queued, 8 = queue full)."""
+        return self.submit_prepared(self.prepare(template, x, out, params, tag), arrival_ns)
+
+    def submit_prepared(self, prepared, arrival_ns: int = 0) -> int:
+        """Queue a request built by prepare() (its checks already done), with
+        its arrival time; returns the library status as submit() does."""
+        r, keep = prepared
+        r.arrival_ns = arrival_ns
+        st = A.lib().xg_server_submit(self.handle, C.byref(r))
+        if st == A.BUSY:
+            return st
+        _check(st)
+        self._keep[r.tag] = keep
+        return st
+
+    @staticmethod
+    def prepare(template: str, x: DeviceArray, out: DeviceArray, params: Sequence[DeviceArray] = (),
+                tag: int = 0):
+        """Check a request and build its descriptor once, ahead of submission."""
         r = A.Request()
         r.tpl = TEMPLATES[template]
         r.dtype = x.dtype_code
@@ -78,13 +96,7 @@ class Server:
         r.in_ = _ptr(x)
         r.out = _ptr(out)
         r.tag = tag
-        r.arrival_ns = arrival_ns
-        st = A.lib().xg_server_submit(self.handle, C.byref(r))
-        if st == A.BUSY:
-            return st
-        _check(st)
-        self._keep[tag] = (x, out, tuple(params))
-        return st
+        return r, (x, out, tuple(params))
 
     def poll(self, cap: int = 4096) -> list:
         buf = (A.Completion * cap)()
