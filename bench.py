@@ -63,6 +63,53 @@ def affine_expr(xt, xv, mean, var, gamma, beta):
     return xt.clip((xv - mean) / xt.sqrt(var + 1e-5) * gamma + beta, -3, 3)
 
 
+def view_variants(np, torch, xt):
+    """The headline's read and write sides apart: copies and the affine clip
+    between the strided view x[:, :, 0:56:2, :] (rows of 56 floats every 112)
+    and a dense [64, 256, 28, 56] array, each over three operand sets (as the
+    timed loop rotates).  Returns [(name, plans, algorithmic bytes)]."""
+    x_h, mean_h, var_h, gamma_h, beta_h = make_inputs(np)
+    params = [xt.from_numpy(v) for v in (mean_h, var_h, gamma_h, beta_h)]
+    view = (xt.all_(), xt.all_(), xt.range_(0, H, 2), xt.all_())
+    dshape = (N, C, H // 2, W)
+    x0 = xt.from_numpy(x_h)
+    d0 = xt.from_numpy(np.ascontiguousarray(x_h[:, :, 0:H:2, :]))
+    sets = []
+    for _ in range(3):
+        x = xt.empty(x_h.shape, "float32")
+        x.storage.copy_(x0.storage)
+        y = xt.zeros(x_h.shape, "float32")
+        d = xt.empty(dshape, "float32")
+        d.storage.copy_(d0.storage)
+        e = xt.zeros(dshape, "float32")
+        sets.append((xt.view(x, *view), xt.view(y, *view), d, e))
+    del x0, d0
+    nb = N * C * (H // 2) * W * 4 * 2
+    out = []
+    for name, make in (
+            ("copy_dense_to_dense", lambda xv, yv, d, e: xt.plan(e, d)),
+            ("copy_view_to_dense", lambda xv, yv, d, e: xt.plan(e, xv)),
+            ("copy_dense_to_view", lambda xv, yv, d, e: xt.plan(yv, d)),
+            ("copy_view_to_view", lambda xv, yv, d, e: xt.plan(yv, xv)),
+            ("affine_clip_dense_to_dense", lambda xv, yv, d, e: xt.plan(e, affine_expr(xt, d, *params))),
+            ("affine_clip_view_to_dense", lambda xv, yv, d, e: xt.plan(e, affine_expr(xt, xv, *params))),
+            ("affine_clip_dense_to_view", lambda xv, yv, d, e: xt.plan(yv, affine_expr(xt, d, *params))),
+            ("affine_clip_view_to_view", lambda xv, yv, d, e: xt.plan(yv, affine_expr(xt, xv, *params)))):
+        out.append((name, [make(*st) for st in sets], nb + (4 * C * 4 if name.startswith("affine") else 0)))
+    return out
+
+
+def view_decomposition(np, torch, xt, steps=24):
+    """Kernel GB/s (algorithmic bytes over HIP-event launch time) of every
+    view variant; the view's read side and write side apart."""
+    res = {}
+    for name, plans, nb in view_variants(np, torch, xt):
+        _, kms = time_plans(torch, plans, steps, 6)
+        res[name] = {"kernel_ms": round(kms, 4), "GBps": round(nb / (kms * 1e-3) / 1e9, 1),
+                     "kernel": xt.last_kernel()}
+    return res
+
+
 def run_steps(torch, plans, steps, warmup):
     """Whole-job wall time of `steps` plan launches (round-robin), nothing but
     the launches between two synchronisations."""
@@ -620,12 +667,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--profile-batch", action="store_true",
-                    help="profiling pass for rocprofv3: the uneven softmax batch, one launch and 3 graph replays")
+                    help="profiling pass for rocprofv3: the uneven softmax batch autotuned as in the timed run "
+                         "(trial launches), then one launch and 3 graph replays")
     ap.add_argument("--profile-reduce", action="store_true",
                     help="profiling pass for rocprofv3: a 256 MiB float32 copy (counter calibration), then "
                          "every reduction and scan of the bench line on [8192, 4096] (float32, its bf16 copy, "
                          "its transpose, int32) and the 2048 x 1024 float64 expression with its row sum and "
                          "column amax, 3 launches each over three input copies")
+    ap.add_argument("--profile-view", action="store_true",
+                    help="profiling pass for rocprofv3: a 256 MiB float32 copy (counter calibration, 3 "
+                         "launches), then each view variant (copies and the affine clip between the strided "
+                         "view and a dense array) 1 + 6 launches rotating over three sets, in a fixed order")
     ap.add_argument("--profile", action="store_true",
                     help="profiling pass for rocprofv3: a 256 MiB float32 copy (counter calibration, 3 "
                          "launches) and the workload kernel as in the timed loop (6 + 12 launches rotating "
@@ -643,7 +695,8 @@ def main():
 
     torch.cuda.set_device(0)
     if args.profile_batch:
-        b, *_rest, elems = make_uneven_batch(np, torch, xt)   # default tuning: no trial launches
+        b, *_rest, elems = make_uneven_batch(np, torch, xt)
+        b.autotune()   # as the timed run: its trial launches come first, the last three launches are replays
         b.launch()
         torch.cuda.synchronize()
         for _ in range(3):
@@ -651,6 +704,24 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"profile_pass": "uneven_batch_softmax_f32", "elements": elems,
                           "algorithmic_bytes_each": 8 * elems, "tuning": b.tuning, "tasks": b.info()["tasks"]}))
+        return
+    if args.profile_view:
+        src = xt.empty((64 << 20,), "float32")
+        src.storage.zero_()
+        dst = xt.empty((64 << 20,), "float32")
+        cp = xt.plan(dst, src)
+        variants = view_variants(np, torch, xt)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            cp.run()
+        order = []
+        for name, plans, nb in variants:
+            for i in range(7):
+                plans[i % 3].run()
+            order.append({"variant": name, "launches": 7, "algorithmic_bytes": nb, "kernel": xt.last_kernel()})
+            torch.cuda.synchronize()
+        print(json.dumps({"profile_pass": "view_decomposition", "copy_bytes_each": 2 * (256 << 20),
+                          "calibration_launches": 3, "order": order}))
         return
     if args.profile_reduce:
         src = xt.empty((64 << 20,), "float32")
@@ -755,6 +826,9 @@ def main():
             A.lib().xg_debug_units_per_thread(0)
         per_upt[str(upt)] = round(nbytes / (ukms * 1e-3) / 1e9, 1)
     extra["GBps_by_units_per_thread"] = per_upt
+    # the strided view's read side and write side apart
+    extra["view_decomposition"] = view_decomposition(np, torch, xt)
+    print("view decomposition done", file=sys.stderr, flush=True)
     # the same workload on the generic interpreter (no compiled specialisation)
     A.lib().xg_debug_force_dynamic(1)
     try:

@@ -276,3 +276,36 @@ def test_functor_view_real_imag():
     assert im.dtype == np.float64 and np.array_equal(im, np.zeros_like(x))
     xt.noalias(xt.real(X)).assign(X * 2.0)
     assert np.array_equal(X.to_numpy(), x * 2.0, equal_nan=True)
+
+
+@pytest.mark.parametrize("w,dtype,n", [(56, "float32", 64), (60, "float32", 64), (28, "int16", 160),
+                                       (28, "float64", 64)])
+def test_strided_rows_with_granule_fill_reads(w, dtype, n):
+    """y[:, :, 0:56:2, :] = x[:, :, 0:56:2, :] + 1 on outputs past 64 MiB whose
+    rows end (and for w = 60 start) inside 64-byte granules: the kernels read
+    the rest of those granules before storing.  The written rows are exact,
+    every byte of the rows left out keeps its (random, NaN-holding) pattern,
+    and the result is bit-identical with the reads switched off."""
+    rng = np.random.default_rng(61)
+    shape = (n, 256, 56, w)
+    if dtype == "int16":
+        x = rng.integers(-30000, 30000, shape, dtype=np.int16)
+    else:
+        x = rng.normal(0, 1, shape).astype(dtype)
+    pattern = rng.integers(0, 256, int(np.prod(shape)) * x.itemsize, dtype=np.uint8).view(x.dtype).reshape(shape)
+    X = xt.from_numpy(x)
+    outs = []
+    try:
+        for fill in (1, 0):
+            A.lib().xg_debug_fill_granules(fill)
+            y = xt.from_numpy(pattern)
+            view = (xt.all_(), xt.all_(), xt.range_(0, 56, 2), xt.all_())
+            xt.noalias(xt.view(y, *view)).assign(xt.cast(xt.view(X, *view) + 1, dtype))
+            sync()
+            outs.append(y.to_numpy())
+    finally:
+        A.lib().xg_debug_fill_granules(1)
+    got = outs[0]
+    assert same_bits(got, outs[1])
+    assert same_bits(got[:, :, 1::2, :], pattern[:, :, 1::2, :])
+    assert same_bits(got[:, :, 0::2, :], (x[:, :, 0::2, :] + 1).astype(dtype))

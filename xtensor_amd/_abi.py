@@ -189,6 +189,7 @@ DEBUG_EXPORTS = {
     "xg_debug_force_dynamic": (None, [C.c_int]),
     "xg_debug_units_per_thread": (None, [C.c_int]),
     "xg_debug_nontemporal": (None, [C.c_int, C.c_int]),
+    "xg_debug_fill_granules": (None, [C.c_int]),
     "xg_debug_jit": (None, [C.c_int]),
     "xg_debug_reduce_waves": (None, [C.c_int]),
     "xg_debug_scan_blocks_per_cu": (None, [C.c_int]),

@@ -885,6 +885,8 @@ void xg_debug_units_per_thread(int upt) { ew_set_units_per_thread(upt); }
 // non-temporal stores of large outputs (both on by default), for plans made
 // after the call.
 void xg_debug_nontemporal(int loads, int stores) { set_nontemporal(loads, stores); }
+// Measurement aid: 0 turns off the granule-fill reads of partly written output granules.
+void xg_debug_fill_granules(int on) { set_fill_granules(on); }
 
 // Run-time compiled specialisations: 0 off, 1 for launches of at least
 // XG_JIT_MIN_ELEMENTS elements (default), 2 for every compact plan.

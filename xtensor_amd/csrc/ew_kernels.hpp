@@ -30,15 +30,17 @@ __device__ __forceinline__ void spec_unit(const P& plan, uint32_t u) {
     }
     if constexpr (RANK >= 2) idx[0] = r;
     const int nvalid = min(V, int(plan.inner - j0));
-    Reg R[NR][V];
-    run_spec<S, 0, V, NR, RANK>(plan, idx, j0, nvalid, R);
     constexpr int odt = S::out_dtype;
     constexpr int esz = dtype_size(odt);
     const int32_t off = coff<RANK, true>(plan.cst[kMaxLeaves], idx, j0);
     char* p = plan.out_ptr + int64_t(off) * esz;
+    uint32_t fill = 0;
+    if constexpr (S::out_mode == LM_CONTIG) fill = fill_read(plan, p, j0, nvalid, esz);
+    Reg R[NR][V];
+    run_spec<S, 0, V, NR, RANK>(plan, idx, j0, nvalid, R);
+    R[S::result_reg][0] |= fill_tie(plan, fill);
     if (S::out_mode == LM_CONTIG && nvalid == V) {
-        if (out_nt(plan)) store_vec<V, true>(p, esz, odt, R[S::result_reg]);
-        else store_vec<V>(p, esz, odt, R[S::result_reg]);
+        store_vec_rt(out_nt(plan), p, esz, odt, R[S::result_reg]);
     } else {
         const int32_t os = plan.cst[kMaxLeaves][RANK - 1];
 #pragma unroll
@@ -66,6 +68,9 @@ __global__ __launch_bounds__(256) void ew_spec_multi_unit_kernel(const EwPlan pl
         uint32_t idx[UPT][4];
         uint32_t j0[UPT];
         int nvalid[UPT];
+        uint32_t fill[UPT];
+        constexpr int odt = S::out_dtype;
+        constexpr int esz = dtype_size(odt);
 #pragma unroll
         for (int k = 0; k < UPT; ++k) {
             const uint32_t u = b + uint32_t(k) * 256u;
@@ -83,18 +88,21 @@ __global__ __launch_bounds__(256) void ew_spec_multi_unit_kernel(const EwPlan pl
                 }
                 if constexpr (RANK >= 2) idx[k][0] = r;
                 nvalid[k] = min(V, int(plan.inner - j0[k]));
+                fill[k] = 0;
+                if constexpr (S::out_mode == LM_CONTIG)
+                    fill[k] = fill_read(plan, plan.out_ptr + int64_t(coff<RANK, true>(plan.cst[kMaxLeaves], idx[k], j0[k])) * esz,
+                                        j0[k], nvalid[k], esz);
                 run_spec<S, 0, V, NR, RANK>(plan, idx[k], j0[k], nvalid[k], R[k]);
+                R[k][S::result_reg][0] |= fill_tie(plan, fill[k]);
             }
         }
-        constexpr int odt = S::out_dtype;
-        constexpr int esz = dtype_size(odt);
 #pragma unroll
         for (int k = 0; k < UPT; ++k) {
             if (nvalid[k] <= 0) continue;
             const int32_t off = coff<RANK, true>(plan.cst[kMaxLeaves], idx[k], j0[k]);
             char* p = plan.out_ptr + int64_t(off) * esz;
             if (S::out_mode == LM_CONTIG && nvalid[k] == V) {
-                store_vec<V>(p, esz, odt, R[k][S::result_reg]);
+                store_vec_rt(out_nt(plan), p, esz, odt, R[k][S::result_reg]);
             } else {
                 const int32_t os = plan.cst[kMaxLeaves][RANK - 1];
 #pragma unroll

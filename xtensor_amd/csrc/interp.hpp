@@ -348,6 +348,12 @@ template <bool NT, class T> __device__ __forceinline__ void st(T* p, T v) {
     else *p = v;
 }
 
+// Placed around the non-temporal store of a branch whose other side stores
+// the same value plainly: an empty asm statement the optimiser cannot move
+// or merge, so it does not hoist or sink the two stores into one and drop
+// the non-temporal hint.  No instruction is emitted.
+__device__ __forceinline__ void nt_store_fence() { __asm__ volatile(""); }
+
 template <bool NT = false>
 __device__ __forceinline__ void store_raw(char* p, int esz, uint64_t raw) {
     switch (esz) {
@@ -398,6 +404,20 @@ __device__ __forceinline__ void store_vec(char* p, int esz, int dt, const Reg (&
     } else {
 #pragma unroll
         for (int v = 0; v < V; ++v) store_raw<NT>(p + v * esz, esz, to_storage(in[v], dt));
+    }
+}
+
+// store_vec with the non-temporal hint chosen at run time; the two forms are
+// kept apart by nt_store_fence, else the optimiser merges them into one plain
+// store
+template <int V>
+__device__ __forceinline__ void store_vec_rt(bool nt, char* p, int esz, int dt, const Reg (&in)[V]) {
+    if (nt) {
+        nt_store_fence();
+        store_vec<V, true>(p, esz, dt, in);
+        nt_store_fence();
+    } else {
+        store_vec<V>(p, esz, dt, in);
     }
 }
 

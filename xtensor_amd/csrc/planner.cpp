@@ -23,6 +23,7 @@ namespace xg {
 
 int g_nt_loads = 1;   // non-temporal loads of streaming leaves
 int g_nt_stores = 1;  // non-temporal stores of large outputs
+int g_fill_granules = 1;  // granule-fill reads for rows that share 64-byte granules with unwritten memory
 
 
 [[noreturn]] void raise(int code, const std::string& msg) { throw Error{code, msg}; }
@@ -662,6 +663,16 @@ EwPlan plan_assign(const xg_expr* e, const xg_tensor* out, int V) {
     };
     p.out_mode = aligned(p.out_ptr, osz, p.out_stride[rank - 1], p.out_stride) ? LM_CONTIG : LM_STRIDED;
     if (g_nt_stores && int64_t(total) * osz >= kNtMinBytes) p.out_mode |= LM_NT;
+    p.fill_zero = 0;
+    if (g_fill_granules && lm_base(p.out_mode) == LM_CONTIG && rank >= 2 &&
+        int64_t(total) * osz >= kNtMinBytes && p.out_stride[rank - 2] != inner) {
+        // rows apart: a row whose bytes do not start and end on 64-byte
+        // boundaries shares its edge granules with memory this launch leaves alone
+        const bool start_ok = reinterpret_cast<uintptr_t>(p.out_ptr) % 64 == 0 && (p.out_stride[rank - 2] * osz) % 64 == 0;
+        bool strides_ok = true;
+        for (int k = 0; k + 2 < rank; ++k) strides_ok = strides_ok && (p.out_stride[k] * osz) % 64 == 0;
+        if (!(start_ok && strides_ok) || (inner * osz) % 64 != 0) p.out_mode |= LM_FILL;
+    }
     for (int pl = 0; pl < p.prog.nleaves; ++pl) {
         int l = p.prog.leaf_src[pl];
         LeafView& lv = p.leaves[pl];
@@ -700,6 +711,8 @@ void set_nontemporal(int loads, int stores) {
     g_nt_loads = loads;
     g_nt_stores = stores;
 }
+
+void set_fill_granules(int on) { g_fill_granules = on; }
 
 void set_gather_leaf(const xg_tensor& t, LeafView* lv) {
     int64_t lo = 0, hi = 0;

@@ -77,6 +77,8 @@ bool streams(const int64_t* stride, const int64_t* shape, int rank, int esz);
 void set_gather_leaf(const xg_tensor& t, LeafView* lv);
 // Switch non-temporal leaf loads / output stores on or off (measurement aid).
 void set_nontemporal(int loads, int stores);
+// Switch the granule-fill reads of partly written output granules on or off (measurement aid).
+void set_fill_granules(int on);
 bool spec_equal(const SpecKey& a, const SpecKey& b);
 
 int64_t numel(const Shape& s);

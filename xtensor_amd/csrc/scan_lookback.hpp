@@ -273,8 +273,7 @@ __global__ __launch_bounds__(256) void scan_lookback_kernel(const LbScan p) {
         const uint32_t j = wbase + uint32_t(u) * SEG + uint32_t(lane * V);
         char* ptr = out + int64_t(j) * oesz;
         if (nv[u] == V) {
-            if (nt_out) store_vec<V, true>(ptr, oesz, odt, o);
-            else store_vec<V>(ptr, oesz, odt, o);
+            store_vec_rt(nt_out, ptr, oesz, odt, o);
         } else {
 #pragma unroll
             for (int v = 0; v < V; ++v)

@@ -260,8 +260,7 @@ __global__ __launch_bounds__(256) void scan_tiles_kernel(const TileScan p) {
                 }
                 char* ptr = obase + (int64_t(row) * p.out_s_len + int64_t(col) * p.out_s_inner) * oesz;
                 if (nv == V) {
-                    if (nt_out) store_vec<V, true>(ptr, oesz, odt, y);
-                    else store_vec<V>(ptr, oesz, odt, y);
+                    store_vec_rt(nt_out, ptr, oesz, odt, y);
                 } else {
 #pragma unroll
                     for (int v = 0; v < V; ++v)
@@ -293,8 +292,7 @@ __global__ __launch_bounds__(256) void scan_tiles_kernel(const TileScan p) {
 #pragma unroll
                 for (int v = 0; v < V; ++v) q[v] = reg_put<T>(sm.tile[c][rr + v]);
                 if (row + uint32_t(V) <= r1) {
-                    if (nt_out) store_vec<V, true>(ptr, oesz, odt, q);
-                    else store_vec<V>(ptr, oesz, odt, q);
+                    store_vec_rt(nt_out, ptr, oesz, odt, q);
                 } else {
 #pragma unroll
                     for (int v = 0; v < V; ++v)

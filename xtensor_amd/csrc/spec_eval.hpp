@@ -58,6 +58,36 @@ __device__ __forceinline__ bool leaf_nt(const RedPlan& p, int l) { return lm_nt(
 __device__ __forceinline__ bool out_nt(const EwPlan& p) { return lm_nt(p.out_mode); }
 __device__ __forceinline__ bool out_nt(const CompactPlan&) { return false; }
 
+// HBM writes whole 64-byte granules: a store that covers part of a granule
+// which is not in the caches makes the memory controller read the granule
+// back and merge.  Measured on the [64,256,56,56] float32 view
+// y[:, :, 0:56:2, :] (rows of 224 bytes every 448, non-temporal stores): a
+// copy from a dense array into the view 54.2 us, 47.3 us when the rest of
+// each row's last granule is read first; the affine clip from the same view
+// of x into it 61.2 us, 54.4 us.  For an output
+// flagged LM_FILL the lane storing a row's last (first) unit reads one word
+// past the row's end (before its start) in the same granule, and ties it into
+// the stored value with plan.fill_zero, which is always 0: the read completes
+// before the store and the stored bytes are unchanged.  The read is issued
+// before the unit's operand loads and its value is used only after them, so
+// it adds no wait of its own.  The word shares a 64-byte granule, hence a
+// mapped page, with a byte of the output; its value is never written
+// anywhere.  One read per unit: a row that is a single unit reads past its end.
+__device__ __forceinline__ uint32_t fill_read(const CompactPlan&, const char*, uint32_t, int, int) { return 0; }
+__device__ __forceinline__ uint32_t fill_read(const EwPlan& plan, const char* p, uint32_t j0, int nvalid, int esz) {
+    if (!lm_fill(plan.out_mode)) return 0;
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    const uintptr_t e = a + uintptr_t(nvalid) * uintptr_t(esz);
+    uintptr_t q = 0;
+    if (j0 + uint32_t(nvalid) >= plan.inner && (e & 63)) q = e & ~uintptr_t(3);
+    else if (j0 == 0 && (a & 63)) q = (a - 1) & ~uintptr_t(3);
+    if (!q) return 0;
+    return *reinterpret_cast<const __attribute__((address_space(1))) uint32_t*>(q);
+}
+// the read's value as a tie into the stored value: always 0
+__device__ __forceinline__ Reg fill_tie(const CompactPlan&, uint32_t) { return 0; }
+__device__ __forceinline__ Reg fill_tie(const EwPlan& plan, uint32_t w) { return Reg(w & plan.fill_zero); }
+
 // FULL as for load_at; NTM: non-temporal leaf loads never (0), always for
 // leaves that are not broadcast (1), or per the plan's leaf flags (2)
 template <class S, int I, int V, int NR, int RANK, bool FULL = false, int NTM = 2, class P>

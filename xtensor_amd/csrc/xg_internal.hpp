@@ -92,10 +92,16 @@ enum LeafMode : uint8_t {
     // exactly one work item (no broadcast reuse) is loaded non-temporally,
     // so the stream does not evict cached lines (dirty ones included) that
     // it will never hit
-    LM_NT = 8
+    LM_NT = 8,
+    // flag on a CONTIG output whose rows end (or start) inside a 64-byte
+    // memory granule that the launch writes only in part (a strided view of
+    // rows apart): before storing a row's last (first) unit, the lane reads
+    // one word of the rest of that granule (see fill_granules)
+    LM_FILL = 16
 };
 XG_HD constexpr int lm_base(int mode) { return mode & 7; }
 XG_HD constexpr bool lm_nt(int mode) { return (mode & LM_NT) != 0; }
+XG_HD constexpr bool lm_fill(int mode) { return (mode & LM_FILL) != 0; }
 // leaves spanning at least this many bytes stream non-temporally
 constexpr int64_t kNtMinBytes = int64_t(64) << 20;
 
@@ -165,8 +171,9 @@ struct LeafView {
 struct EwPlan {
     int32_t rank;
     int32_t out_dtype;
-    int32_t out_mode;        // LM_CONTIG or LM_STRIDED
+    int32_t out_mode;        // LM_CONTIG or LM_STRIDED, | LM_NT, | LM_FILL
     int32_t index64;         // 1 if any offset needs 64-bit arithmetic
+    uint32_t fill_zero;      // always 0: ties a granule-fill read into the stored value
     int64_t shape[kMaxRank];
     FastDiv div[kMaxRank];   // div[k] divides by shape[k] (outer dims)
     uint32_t inner;          // shape[rank-1]
