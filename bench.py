@@ -63,6 +63,29 @@ def affine_expr(xt, xv, mean, var, gamma, beta):
     return xt.clip((xv - mean) / xt.sqrt(var + 1e-5) * gamma + beta, -3, 3)
 
 
+def time_plans_region(torch, plans, steps, warmup):
+    """Run `steps` plans round-robin back to back; returns (wall seconds, mean
+    kernel ms) where the kernel time is one pair of HIP events around the
+    whole timed region on the launch stream, divided by `steps`.  Launches
+    run back to back (a kernel trace shows ~0.1 us between them), so this is
+    the launches' average duration as a kernel trace reports it; an event
+    pair around every launch would add the event packets' own time (measured
+    4-7 us a launch on the headline)."""
+    for i in range(warmup):
+        plans[i % len(plans)].run()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for i in range(steps):
+        plans[i % len(plans)].run()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, e0.elapsed_time(e1) / steps
+
+
 def view_variants(np, torch, xt):
     """The headline's read and write sides apart: copies and the affine clip
     between the strided view x[:, :, 0:56:2, :] (rows of 56 floats every 112)
@@ -100,27 +123,14 @@ def view_variants(np, torch, xt):
 
 
 def view_decomposition(np, torch, xt, steps=24):
-    """Kernel GB/s (algorithmic bytes over HIP-event launch time) of every
-    view variant; the view's read side and write side apart."""
+    """Kernel GB/s (algorithmic bytes over the launches' average duration) of
+    every view variant; the view's read side and write side apart."""
     res = {}
     for name, plans, nb in view_variants(np, torch, xt):
-        _, kms = time_plans(torch, plans, steps, 6)
+        _, kms = time_plans_region(torch, plans, steps, 6)
         res[name] = {"kernel_ms": round(kms, 4), "GBps": round(nb / (kms * 1e-3) / 1e9, 1),
                      "kernel": xt.last_kernel()}
     return res
-
-
-def run_steps(torch, plans, steps, warmup):
-    """Whole-job wall time of `steps` plan launches (round-robin), nothing but
-    the launches between two synchronisations."""
-    for i in range(warmup):
-        plans[i % len(plans)].run()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(steps):
-        plans[i % len(plans)].run()
-    torch.cuda.synchronize()
-    return time.perf_counter() - t0
 
 
 def time_plans(torch, plans, steps, warmup):
@@ -801,17 +811,17 @@ def main():
         for _ in range(3):
             cp.run()
         # the timed loop's steady state: launches rotating over the three sets
-        time_plans(torch, plans, 12, 6)
+        time_plans_region(torch, plans, 12, 6)
         torch.cuda.synchronize()
         print(json.dumps({"profile_pass": True, "copy_bytes_each": 2 * (256 << 20),
                           "workload_algorithmic_bytes": algorithmic_bytes(), "kernel": kernel_name}))
         return
 
     nbytes = algorithmic_bytes()
-    # kernel time: HIP events around every launch on the launch stream;
-    # whole-job value: the same launches timed end to end without the events
-    _, kms = time_plans(torch, plans, args.steps, args.warmup)
-    wall = run_steps(torch, plans, args.steps, args.warmup)
+    # kernel time: HIP events around the timed region on the launch stream,
+    # over the launches; whole-job value: the same region on the host clock
+    # (both synchronisations inside it)
+    wall, kms = time_plans_region(torch, plans, args.steps, args.warmup)
     value = nbytes * args.steps / wall / 1e9
     achieved = nbytes / (kms * 1e-3) / 1e9
 
@@ -821,7 +831,7 @@ def main():
     for upt in (1, 4):
         A.lib().xg_debug_units_per_thread(upt)
         try:
-            _, ukms = time_plans(torch, plans, max(10, args.steps // 2), 3)
+            _, ukms = time_plans_region(torch, plans, max(10, args.steps // 2), 3)
         finally:
             A.lib().xg_debug_units_per_thread(0)
         per_upt[str(upt)] = round(nbytes / (ukms * 1e-3) / 1e9, 1)

@@ -150,3 +150,45 @@ def test_float_moments_and_norm_in_pieces():
     assert abs(got - var_ref ** 0.5) / var_ref ** 0.5 <= 1e-6
     del f, F, fd
     torch.cuda.empty_cache()
+
+
+def test_scans_past_2_31():
+    """cumsum / cumprod along an axis of more than 2^31 elements, and the
+    flattened cumsum of such an array: pieces of 2^30 are scanned in turn and
+    from the second on the previous piece's last value is folded in.  Inputs
+    with closed-form scans (steps of 1 at every third element, signs flipping
+    every 2^20 elements), checked on every element with torch arithmetic."""
+    import torch
+
+    def step(msg):
+        print("  step:", msg, flush=True)
+
+    n = (1 << 31) + (1 << 21) + 3
+    i = torch.arange(n, device="cuda", dtype=torch.int64)
+    x = (i % 3 == 0).to(torch.int32)
+    ref = (i // 3 + 1).to(torch.int32)  # cumsum: the count of multiples of 3 up to i
+    X = wrap(x, (n,))
+    for axis in (0, None):
+        step(f"cumsum int32 over {n} elements, axis {axis}")
+        got = xt.cumsum(X, axis)
+        assert got.shape == (n,) and got.dtype == "int32"
+        assert torch.equal(got.storage.view(torch.int32), ref), axis
+        del got
+    del ref, x, X
+    step("cumprod of signs")
+    s = torch.where(i % (1 << 20) == 0, -1, 1).to(torch.int32)
+    ref = torch.where((i >> 20) % 2 == 0, -1, 1).to(torch.int32)  # one flip at each multiple of 2^20
+    got = xt.cumprod(wrap(s, (n,)), 0)
+    assert torch.equal(got.storage.view(torch.int32), ref)
+    del got, s, ref, i
+    torch.cuda.empty_cache()
+    step("cumsum along axis 1 of [2, 2^31 + 8]")
+    c = (1 << 31) + 8  # rows of whole 16-byte vectors, 64-bit row offsets
+    j = torch.arange(c, device="cuda", dtype=torch.int64)
+    x2 = torch.stack([(j % 3 == 0), (j % 5 == 0)]).to(torch.int32)
+    got = xt.cumsum(wrap(x2, (2, c)), 1)
+    g = got.storage.view(torch.int32).view(2, c)
+    assert torch.equal(g[0], (j // 3 + 1).to(torch.int32))
+    assert torch.equal(g[1], (j // 5 + 1).to(torch.int32))
+    del got, g, x2, j
+    torch.cuda.empty_cache()

@@ -655,6 +655,99 @@ xg_status xg_scan_info(const xg_expr* e, const xg_scan_desc* s, int32_t* dtype, 
     });
 }
 
+// A scan along an axis of 2^31 elements or more, cut along that axis into
+// pieces of 2^30: each piece is scanned into its slice of the output, and
+// from the second piece on the previous piece's last slice is folded in,
+// out_q = out_q + last_{q-1} (cumprod: *), one elementwise launch with that
+// slice broadcast along the axis, in the output's type (for integers the
+// same residues as one sequential scan; for floats one more rounding per
+// element past the first piece).
+struct ScanPieces {
+    const xg_expr* e;
+    const xg_scan_desc* s;
+    const xg_tensor* out;
+    int R = 0, d = -1;
+    int64_t L = 0;
+    std::vector<xg_tensor> leaves;
+    xg_expr pe{};
+    static constexpr int64_t kPiece = int64_t(1) << 30;
+
+    // false: no piece applies (the axis is shorter, or a gather)
+    bool init(const xg_expr* e_, const xg_scan_desc* s_, const xg_tensor* out_) {
+        e = e_;
+        s = s_;
+        out = out_;
+        for (int i = 0; i < e->n_nodes; ++i)
+            if (e->nodes[i].op == XG_OP_GATHER) return false;
+        const Shape shape = analyse(e).shape;
+        R = int(shape.size());
+        d = s->axis < 0 ? s->axis + R : s->axis;
+        if (d < 0 || d >= R || out->ndim != R) return false;
+        L = shape[size_t(d)];
+        if (L < (int64_t(1) << 31)) return false;
+        leaves.assign(e->leaves, e->leaves + e->n_leaves);
+        pe = *e;
+        pe.leaves = leaves.data();
+        return true;
+    }
+    // the piece starting at s0: leaves and output sliced along d
+    xg_tensor select(int64_t s0) {
+        const int64_t len = std::min(kPiece, L - s0);
+        for (int l = 0; l < e->n_leaves; ++l) {
+            const xg_tensor& t = e->leaves[l];
+            xg_tensor& pl = leaves[size_t(l)];
+            pl = t;
+            const int dl = d - (R - t.ndim);  // the leaf's axis aligned with d
+            if (dl >= 0 && t.shape[dl] != 1) {
+                pl.offset += s0 * t.strides[dl];
+                pl.shape[dl] = len;
+            }
+        }
+        xg_tensor oq = *out;
+        oq.offset += s0 * out->strides[d];
+        oq.shape[d] = len;
+        return oq;
+    }
+    // the largest scan workspace of a piece (plans only, no launch)
+    size_t workspace_bytes(int cus) {
+        size_t m = 0;
+        for (int64_t s0 = 0; s0 < L; s0 += kPiece) {
+            xg_tensor oq = select(s0);
+            m = std::max(m, scan_workspace_bytes(plan_scan(&pe, s, &oq, 4, cus)));
+        }
+        return m;
+    }
+    void run(char* ws, size_t ws_bytes, hipStream_t st, const char** used) {
+        const int cus = device_cus();
+        for (int64_t s0 = 0; s0 < L; s0 += kPiece) {
+            xg_tensor oq = select(s0);
+            ScanPlan p = plan_scan(&pe, s, &oq, 4, cus);
+            if (scan_workspace_bytes(p) > ws_bytes) raise(XG_INVALID_ARGUMENT, "internal: scan workspace grew after sizing");
+            check_hip(launch_scan(p, 4, ws, st, used), "scan launch");
+            if (s0 == 0) continue;
+            xg_tensor prev = *out;  // the previous piece's last slice, extent 1 along d
+            prev.offset += (s0 - 1) * out->strides[d];
+            prev.shape[d] = 1;
+            xg_tensor cl[2] = {oq, prev};
+            xg_node nodes[3]{};
+            nodes[0].op = XG_OP_LEAF;
+            nodes[0].index = 0;
+            nodes[1].op = XG_OP_LEAF;
+            nodes[1].index = 1;
+            nodes[2].op = s->op == XG_SCAN_CUMPROD ? XG_OP_MULTIPLIES : XG_OP_PLUS;
+            nodes[2].nargs = 2;
+            nodes[2].args[0] = 0;
+            nodes[2].args[1] = 1;
+            xg_expr ce{};
+            ce.n_nodes = 3;
+            ce.n_leaves = 2;
+            ce.nodes = nodes;
+            ce.leaves = cl;
+            assign_launch(&ce, &oq, st, nullptr);
+        }
+    }
+};
+
 xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, xg_stream_t stream) {
     return guard([&] {
         if (!s || !out) raise(XG_INVALID_ARGUMENT, "null argument");
@@ -664,8 +757,18 @@ xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out,
         scan_result(e, s, &dt, &rs);
         const int V = 4;
         if (s->axis != XG_AXIS_NONE) {
-            ScanPlan p = plan_scan(e, s, out, V, device_cus());
+            ScanPlan p;
             const char* used = nullptr;
+            try {
+                p = plan_scan(e, s, out, V, device_cus());
+            } catch (const Error& err) {
+                ScanPieces sp;
+                if (err.code != XG_UNSUPPORTED || !sp.init(e, s, out)) throw;
+                const size_t wsb = sp.workspace_bytes(device_cus());
+                sp.run(wsb ? static_cast<char*>(workspace(wsb, st)) : nullptr, wsb, st, &used);
+                note_kernel(used);
+                return;
+            }
             check_hip(launch_scan(p, V, workspace(scan_workspace_bytes(p), st), st, &used), "scan launch");
             note_kernel(used);
             return;
@@ -712,6 +815,29 @@ xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out,
         xg_scan_desc s1 = *s;
         s1.axis = 0;
         const int cus = device_cus();
+        if (n >= (int64_t(1) << 31)) {  // past 32-bit indexing: the flat row in pieces
+            ScanPieces sp;
+            if (!sp.init(&fe, &s1, out)) raise(XG_UNSUPPORTED, "flattened scan too large");
+            const size_t wsb = sp.workspace_bytes(cus);  // sized on the placeholder address
+            char* block = tmp_bytes + wsb ? static_cast<char*>(workspace(tmp_bytes + wsb, st)) : nullptr;
+            if (!lone) {
+                xg_tensor tmp{};
+                tmp.data = block;
+                tmp.dtype = dt;
+                tmp.ndim = int32_t(info.shape.size());
+                for (int64_t k = tmp.ndim - 1, stv = 1; k >= 0; --k) {
+                    tmp.shape[k] = info.shape[size_t(k)];
+                    tmp.strides[k] = stv;
+                    stv *= info.shape[size_t(k)];
+                }
+                assign_launch(e, &tmp, st, nullptr);
+                flat.data = block;
+            }
+            const char* used = nullptr;
+            sp.run(wsb ? block + tmp_bytes : nullptr, wsb, st, &used);
+            note_kernel(used);
+            return;
+        }
         ScanPlan p = plan_scan(&fe, &s1, out, V, cus);
         const size_t wsb = scan_workspace_bytes(p);
         char* block = tmp_bytes + wsb ? static_cast<char*>(workspace(tmp_bytes + wsb, st)) : nullptr;

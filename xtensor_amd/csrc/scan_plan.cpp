@@ -187,7 +187,9 @@ size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 void set_scan_blocks_per_cu(int n) { g_tile_blocks_per_cu = n > 0 ? n : 8; }
 
 bool scan_direct_ok(const ScanPlan& p) {
-    if (p.layout != SL_ROWS || p.index64 || p.rank < 1 || p.rank > 2 || p.acc < 0 || p.acc >= AK_NUM) return false;
+    // rows are addressed with 64-bit offsets and elements within a row with
+    // 32-bit ones (a row is shorter than 2^31), so 64-bit plans qualify too
+    if (p.layout != SL_ROWS || p.rank < 1 || p.rank > 2 || p.acc < 0 || p.acc >= AK_NUM) return false;
     if (p.rank == 1 && p.rows != 1) return false;
     if (p.prog.nops != 1 || p.prog.ops[0].op != MO_LOAD || p.prog.nleaves != 1 || p.prog.leaf_gather[0]) return false;
     const LeafView& l = p.leaves[0];
