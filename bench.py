@@ -133,23 +133,6 @@ def view_decomposition(np, torch, xt, steps=24):
     return res
 
 
-def time_plans(torch, plans, steps, warmup):
-    """Run `steps` plans round-robin; returns (wall seconds, mean kernel ms)."""
-    for i in range(warmup):
-        plans[i % len(plans)].run()
-    torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
-    t0 = time.perf_counter()
-    for i in range(steps):
-        ev[i][0].record()
-        plans[i % len(plans)].run()
-        ev[i][1].record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kms = sum(a.elapsed_time(b) for a, b in ev) / steps
-    return wall, kms
-
-
 def _cpu_info():
     """CPU model and logical CPU count of this host (the GPU box's host cores)."""
     model = "unknown"
@@ -405,7 +388,7 @@ def reductions(np, torch, xt, steps=12, copies=3):
                 o = xt.empty(xt.expr_info(e)[1], dtype or xt.dtype_name(xt.expr_info(e)[0]))
             plans.append(xt.plan(o, e))
             outs.append(o)
-        _, kms = time_plans(torch, plans, steps, len(plans))
+        _, kms = time_plans_region(torch, plans, steps, len(plans))
         ref = np.asarray(ref, np.float64)
         err = 0.0
         for o in outs:
@@ -438,7 +421,7 @@ def reductions(np, torch, xt, steps=12, copies=3):
     # cumsum along axis 1 (scan plans, one output per input copy)
     css = [xt.empty((R, Cc), "float32") for _ in dX]
     pcs = [xt.plan(cs, xt.Scan("cumsum", d, 1)) for cs, d in zip(css, dX)]
-    _, kms = time_plans(torch, pcs, steps, len(pcs))
+    _, kms = time_plans_region(torch, pcs, steps, len(pcs))
     ref = np.cumsum(X64, axis=1)
     got = css[0].to_numpy().astype(np.float64)
     out["cumsum_axis1_f32"] = {"kernel_ms": round(kms, 4), "GBps": round(2 * nb / (kms * 1e-3) / 1e9, 1),
@@ -452,7 +435,7 @@ def reductions(np, torch, xt, steps=12, copies=3):
         to max(|ref|, 1)"""
         outs = [xt.empty(v.shape, "float32") for v in views]
         plans = [xt.plan(o, xt.Scan("cumsum", v, axis)) for o, v in zip(outs, views)]
-        _, skms = time_plans(torch, plans, steps, len(plans))
+        _, skms = time_plans_region(torch, plans, steps, len(plans))
         kname = xt.last_kernel()
         got = outs[0].to_numpy().astype(np.float64)
         out[name] = {"kernel_ms": round(skms, 4), "GBps": round(2 * nb / (skms * 1e-3) / 1e9, 1),
@@ -474,7 +457,7 @@ def reductions(np, torch, xt, steps=12, copies=3):
     da, db = xt.from_numpy(a), xt.from_numpy(b)
     c = xt.empty((2048, 1024), "float64")
     pc = xt.plan(c, da * db + xt.exp(-da * da))
-    _, kms = time_plans(torch, [pc], steps, 2)
+    _, kms = time_plans_region(torch, [pc], steps, 2)
     kname = xt.last_kernel()
     # the host libm's exp (math.exp), element by element, as xtensor's
     # non-SIMD path calls std::exp; the device exp's ULP histogram against it
@@ -517,7 +500,48 @@ def reductions(np, torch, xt, steps=12, copies=3):
     graph_ms = per_step(graph.replay)
     out["pipeline_c_sum_amax_f64"] = {"launches": len(seq), "eager_ms_per_step": round(eager_ms, 4),
                                       "graph_replay_ms_per_step": round(graph_ms, 4)}
+    del seq, graph, pc, c, da, db
+    out.update(fused_expr_reduce_large(np, torch, xt, steps))
     return out
+
+
+def fused_expr_reduce_large(np, torch, xt, steps, rows=16384, cols=8192):
+    """The config-1 expression fused into its axis reductions far past the
+    MALL: float64 a [16384, 8192] (1 GiB, uniform(-1, 1), seed 0) and b [8192];
+    sum(a*b + exp(-a*a), {1}) and amax(a*b + exp(-a*a), {0}), each one pass
+    over a (c is never written), over three copies of a.  Error against the
+    float64 evaluation in numpy (exp from numpy, sums pairwise)."""
+    r = np.random.default_rng(0)
+    a = r.uniform(-1, 1, (rows, cols))
+    b = r.uniform(-1, 1, cols)
+    da0 = xt.from_numpy(a)
+    das = [da0]
+    for _ in range(2):
+        d = xt.empty(a.shape, "float64")
+        d.storage.copy_(da0.storage)
+        das.append(d)
+    db = xt.from_numpy(b)
+    c = a * b + np.exp(-a * a)
+    res = {}
+    for name, make, oshape, ref in (
+            ("fused_sum_axis1_mul_add_exp_f64_16384x8192", lambda d: xt.sum(d * db + xt.exp(-d * d), [1]), (rows,),
+             c.sum(1)),
+            ("fused_amax_axis0_mul_add_exp_f64_16384x8192", lambda d: xt.amax(d * db + xt.exp(-d * d), [0]), (cols,),
+             c.max(0))):
+        outs = [xt.empty(oshape, "float64") for _ in das]
+        plans = [xt.plan(o, make(d)) for o, d in zip(outs, das)]
+        _, kms = time_plans_region(torch, plans, steps, len(plans))
+        kname = xt.last_kernel()
+        got = outs[0].to_numpy()
+        nbytes = a.nbytes + b.nbytes + outs[0].size * 8
+        res[name] = {"kernel_ms": round(kms, 4), "GBps": round(nbytes / (kms * 1e-3) / 1e9, 1),
+                     "frac": round(nbytes / (kms * 1e-3) / 8e12, 4),
+                     "max_rel_err_vs_fp64": float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300))),
+                     "kernel": kname}
+        del outs, plans
+    del das, da0
+    torch.cuda.empty_cache()
+    return res
 
 
 SERVE_TEMPLATES = {  # template -> dtypes it is defined for
@@ -843,7 +867,7 @@ def main():
     A.lib().xg_debug_force_dynamic(1)
     try:
         dplans = [xt.plan(xt.view(y, *view), affine_expr(xt, xt.view(x, *view), *params)) for x, y in sets]
-        _, dkms = time_plans(torch, dplans, max(5, args.steps // 5), 2)
+        _, dkms = time_plans_region(torch, dplans, max(5, args.steps // 5), 2)
         extra["generic_interpreter_GBps"] = round(nbytes / (dkms * 1e-3) / 1e9, 1)
     finally:
         A.lib().xg_debug_force_dynamic(0)
@@ -858,7 +882,7 @@ def main():
     torch.cuda.synchronize()
     compile_s = time.perf_counter() - t0
     jkernel = xt.last_kernel()
-    _, jkms = time_plans(torch, jplans, max(10, args.steps // 2), 3)
+    _, jkms = time_plans_region(torch, jplans, max(10, args.steps // 2), 3)
     extra["runtime_compiled_f32_literal"] = {"kernel": jkernel, "GBps": round(nbytes / (jkms * 1e-3) / 1e9, 1),
                                              "first_call_s": round(compile_s, 2)}
     del jplans
@@ -867,19 +891,19 @@ def main():
     # rotating over the three sets, through the compiled copy program)
     touched = N * C * (H // 2) * W
     cplans = [xt.plan(xt.view(y, *view), xt.view(x, *view)) for x, y in sets]
-    _, ckms = time_plans(torch, cplans, max(10, args.steps // 2), 3)
+    _, ckms = time_plans_region(torch, cplans, max(10, args.steps // 2), 3)
     extra["view_copy_GBps"] = round(8 * touched / (ckms * 1e-3) / 1e9, 1)
     extra["view_copy_kernel"] = xt.last_kernel()
     fplans = [xt.plan(xt.DeviceArray(y.storage, "float32", (touched,), None, 0),
                       xt.DeviceArray(x.storage, "float32", (touched,), None, 0)) for x, y in sets]
-    _, fkms = time_plans(torch, fplans, max(10, args.steps // 2), 3)
+    _, fkms = time_plans_region(torch, fplans, max(10, args.steps // 2), 3)
     extra["contiguous_copy_same_bytes_GBps"] = round(8 * touched / (fkms * 1e-3) / 1e9, 1)
     del cplans, fplans
     # int32 variant: y = (x*3 + (x>>2)) & 0xFFFF over the full array
     xi = xt.from_numpy(np.random.default_rng(1).integers(-2**20, 2**20, (N, C, H, W), dtype=np.int32))
     yi = xt.empty((N, C, H, W), "int32")
     ip = [xt.plan(yi, (xi * 3 + (xi >> 2)) & 0xFFFF)]
-    _, ikms = time_plans(torch, ip, max(5, args.steps // 5), 2)
+    _, ikms = time_plans_region(torch, ip, max(5, args.steps // 5), 2)
     extra["int32_bitmix_GBps"] = round(N * C * H * W * 8 / (ikms * 1e-3) / 1e9, 1)
     del xi, yi, ip
     extra["reductions_8192x4096"] = reductions(np, torch, xt)
