@@ -498,9 +498,23 @@ def reductions(np, torch, xt, steps=12, copies=3):
     eager_ms = per_step(lambda: [p.run() for p in seq])
     graph = xt.PlanGraph(seq)
     graph_ms = per_step(graph.replay)
+    # the same three results from one pass over c (xg_plan_assign_reduce2d):
+    # the fused kernel writes c and the row sums, a second launch merges the
+    # column maxima of its row bands
+    pf = xt.plan(c, da * db + xt.exp(-da * da), rows=("sum", s1), cols=("amax", m1))
+    pf.run()
+    torch.cuda.synchronize()
+    fused_ok = (np.array_equal(c.to_numpy(), got_c) and np.array_equal(m1.to_numpy(), got_c.max(0)) and
+                float(np.max(np.abs(s1.to_numpy() - c_ref.sum(1)) / np.abs(c_ref.sum(1)))) <= 1e-14)
+    fused_ms = per_step(pf.run)
+    fgraph = xt.PlanGraph([pf])
+    fgraph_ms = per_step(fgraph.replay)
     out["pipeline_c_sum_amax_f64"] = {"launches": len(seq), "eager_ms_per_step": round(eager_ms, 4),
-                                      "graph_replay_ms_per_step": round(graph_ms, 4)}
-    del seq, graph, pc, c, da, db
+                                      "graph_replay_ms_per_step": round(graph_ms, 4),
+                                      "one_pass_ms_per_step": round(fused_ms, 4),
+                                      "one_pass_graph_replay_ms_per_step": round(fgraph_ms, 4),
+                                      "one_pass_matches": bool(fused_ok)}
+    del seq, graph, pc, c, da, db, pf, fgraph
     out.update(fused_expr_reduce_large(np, torch, xt, steps))
     return out
 
@@ -803,7 +817,9 @@ def main():
         s1, m1 = xt.empty((2048,), "float64"), xt.empty((1024,), "float64")
         for name, p in (("mul_add_exp_f64_2048x1024", xt.plan(c, a * b + xt.exp(-a * a))),
                         ("sum_axis1_of_c_f64", xt.plan(s1, xt.sum(c, [1]))),
-                        ("amax_axis0_of_c_f64", xt.plan(m1, xt.amax(c, [0])))):
+                        ("amax_axis0_of_c_f64", xt.plan(m1, xt.amax(c, [0]))),
+                        ("one_pass_c_sum_amax_f64", xt.plan(c, a * b + xt.exp(-a * a), rows=("sum", s1),
+                                                            cols=("amax", m1)))):
             for _ in range(3):
                 p.run()
             names[name] = xt.last_kernel()

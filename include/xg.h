@@ -320,6 +320,19 @@ xg_status xg_scan_info(const xg_expr* e, const xg_scan_desc* s, int32_t* dtype, 
 /* out = cumsum/cumprod(e, axis). */
 xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, xg_stream_t stream);
 
+/* out = e for a 2-D row-major output, and in the same pass over the stored
+ * values the reduction `row_red` of every row (axes {1}) into `row_out` and
+ * `col_red` of every column (axes {0}) into `col_out`; either pair may be NULL.
+ * The results equal xg_assign(e, out) followed by xg_reduce of `out` with
+ * each descriptor (within the reductions' stated error: the fold order
+ * differs), and the call falls back to exactly those calls where the fused
+ * kernel does not apply (see DESIGN).  No counterpart in xtensor, where
+ * `c = e; s = sum(c, {1}); m = amax(c, {0})` (core/xassign.hpp:208-226,
+ * reducers/xreducer.hpp:1778-1828) reads c back twice. */
+xg_status xg_assign_reduce2d(const xg_expr* e, const xg_tensor* out, const xg_reduce_desc* row_red,
+                             const xg_tensor* row_out, const xg_reduce_desc* col_red, const xg_tensor* col_out,
+                             xg_stream_t stream);
+
 /* Assignment into an index view: target's element at offset offsets[i] (from
  * the target's own offset) = values[i], for the 1-d integer expression
  * `offsets` and `values` broadcast to its length.  Replaces the assignment of
@@ -367,6 +380,12 @@ typedef struct xg_plan_s* xg_plan_t;
 xg_status xg_plan_assign(const xg_expr* e, const xg_tensor* out, xg_plan_t* plan);
 xg_status xg_plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, xg_plan_t* plan);
 xg_status xg_plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, xg_plan_t* plan);
+/* xg_assign_reduce2d prepared: XG_UNSUPPORTED when the single pass does not
+ * apply to these descriptors (then plan the assignment and the reductions
+ * apart). */
+xg_status xg_plan_assign_reduce2d(const xg_expr* e, const xg_tensor* out, const xg_reduce_desc* row_red,
+                                  const xg_tensor* row_out, const xg_reduce_desc* col_red, const xg_tensor* col_out,
+                                  xg_plan_t* plan);
 /* Bytes of device workspace the plan needs (0 for most elementwise plans). */
 size_t xg_plan_workspace_size(xg_plan_t plan);
 /* Enqueue the plan on `stream`; `workspace` must hold xg_plan_workspace_size

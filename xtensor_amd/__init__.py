@@ -8,7 +8,7 @@ ABI of `include/xg.h`; there is no CPU fallback.
 """
 
 from .core import (  # noqa: F401
-    BroadcastError, DeviceArray, DeviceError, Expr, Func, Reducer, Scalar, Scan, assign_checked,
+    BroadcastError, DeviceArray, DeviceError, Expr, Func, Reducer, Scalar, Scan, assign_checked, assign_reduce,
     Filtration, IndexView, abs, acos, all_, amax, amin, argwhere, asarray, asin, assign, atan, atan2, broadcast,
     cast, cbrt, ceil, clip, count_nonzero, filter, filtration, functor_view, imag, index_view, masked_view, real,
     cos, cosh, cumprod, cumsum, dtype_code, dtype_name, dump_npy, empty, equal, erf, eval, exp, exp2, expm1,

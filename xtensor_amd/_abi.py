@@ -149,6 +149,8 @@ EXPORTS = {
     "xg_assign": (C.c_int, [_P(Expr), _P(Tensor), C.c_void_p]),
     "xg_reduce_info": (C.c_int, [_P(Expr), _P(ReduceDesc), _P(C.c_int32), _P(C.c_int32), _P(C.c_int64)]),
     "xg_reduce": (C.c_int, [_P(Expr), _P(ReduceDesc), _P(Tensor), C.c_void_p]),
+    "xg_assign_reduce2d": (C.c_int, [_P(Expr), _P(Tensor), _P(ReduceDesc), _P(Tensor), _P(ReduceDesc), _P(Tensor),
+                                     C.c_void_p]),
     "xg_reduce_noalloc": (C.c_int, [_P(Expr), _P(ReduceDesc), _P(Tensor), C.c_void_p]),
     "xg_reduce_noalloc_max_bytes": (C.c_size_t, []),
     "xg_scan_info": (C.c_int, [_P(Expr), _P(ScanDesc), _P(C.c_int32), _P(C.c_int32), _P(C.c_int64)]),
@@ -159,6 +161,8 @@ EXPORTS = {
     "xg_npy_load": (C.c_int, [C.c_char_p, _P(Tensor), C.c_void_p]),
     "xg_npy_dump": (C.c_int, [C.c_char_p, _P(Tensor), C.c_void_p]),
     "xg_plan_assign": (C.c_int, [_P(Expr), _P(Tensor), _P(C.c_void_p)]),
+    "xg_plan_assign_reduce2d": (C.c_int, [_P(Expr), _P(Tensor), _P(ReduceDesc), _P(Tensor), _P(ReduceDesc),
+                                          _P(Tensor), _P(C.c_void_p)]),
     "xg_plan_reduce": (C.c_int, [_P(Expr), _P(ReduceDesc), _P(Tensor), _P(C.c_void_p)]),
     "xg_plan_scan": (C.c_int, [_P(Expr), _P(ScanDesc), _P(Tensor), _P(C.c_void_p)]),
     "xg_plan_workspace_size": (C.c_size_t, [C.c_void_p]),

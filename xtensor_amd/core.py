@@ -777,6 +777,49 @@ def assign(out, e):
     return out
 
 
+def assign_reduce(out: DeviceArray, e, rows=None, cols=None) -> DeviceArray:
+    """noalias(out) = e for a 2-D row-major `out`, and in the same pass over
+    the stored values the reduction of every row and / or every column:
+    rows=("sum", s) puts sum(out, {1}) into s, cols=("amax", m) puts
+    amax(out, {0}) into m (a third item, a dict, passes keep_dims= /
+    initial= / dtype=).
+    The results are those of the three statements
+        out = e; s = sum(out, {1}); m = amax(out, {0})
+    (core/xassign.hpp:208-226, reducers/xreducer.hpp:1778-1828), which the
+    library runs instead where the single pass does not apply.  Returns out."""
+    e = _wrap(e)
+    s, args, _ = _assign_reduce_args(out, e, rows, cols)
+    _check(A.lib().xg_assign_reduce2d(s.ptr(), *args, _stream()))
+    return out
+
+
+def _assign_reduce_args(out, e, rows, cols):
+    """The serialized expression and the C arguments (out, row desc, row out,
+    col desc, col out) of xg_assign_reduce2d / xg_plan_assign_reduce2d, and
+    the Python objects they point into."""
+    def desc(spec, axis):
+        if spec is None:
+            return None, None
+        kind, dst = spec[0], spec[1]
+        kw = dict(spec[2]) if len(spec) > 2 else {}
+        r = Reducer(kind, out, axes=[axis], keep_dims=kw.get("keep_dims", False), initial=kw.get("initial"),
+                    dtype=kw.get("dtype"))
+        if not isinstance(dst, DeviceArray):
+            raise TypeError("a reduction's output must be a DeviceArray")
+        return r.to_abi(), dst
+
+    rd, ro = desc(rows, 1)
+    cd, co = desc(cols, 0)
+    _require_device(_leaves(e) + [out] + [x for x in (ro, co) if x is not None])
+    s = serialize(e, reducer_fn=_eval_reducer)
+    t = out.to_abi()
+    rt = ro.to_abi() if ro is not None else None
+    ct = co.to_abi() if co is not None else None
+    ref = lambda x: C.byref(x) if x is not None else None  # noqa: E731
+    keep = [t, rd, rt, cd, ct, ro, co]
+    return s, (ref(t), ref(rd), ref(rt), ref(cd), ref(ct)), keep
+
+
 def _same_view(a: DeviceArray, b: DeviceArray) -> bool:
     return (a.storage is b.storage and a.offset == b.offset and a.shape == b.shape
             and all(s1 == s2 for n, s1, s2 in zip(a.shape, a.strides, b.strides) if n > 1))
@@ -1085,12 +1128,16 @@ class Plan:
     keeps its operands alive and reads whatever they hold at run time.
     """
 
-    def __init__(self, out: DeviceArray, e):
+    def __init__(self, out: DeviceArray, e, rows=None, cols=None):
         e = e if isinstance(e, Scan) else _wrap(e)
         self._keep = [out, e]
         h = C.c_void_p()
         L = A.lib()
-        if isinstance(e, Scan):
+        if rows is not None or cols is not None:  # out = e with its row / column reductions in one pass
+            s, args, keep = _assign_reduce_args(out, e, rows, cols)
+            self._keep += keep
+            _check(L.xg_plan_assign_reduce2d(s.ptr(), *args, C.byref(h)))
+        elif isinstance(e, Scan):
             _require_device(_leaves(e.expr) + [out])
             s = serialize(e.expr, reducer_fn=_eval_reducer)
             d = e.to_abi()
@@ -1125,8 +1172,10 @@ class Plan:
             pass
 
 
-def plan(out: DeviceArray, e) -> Plan:
-    return Plan(out, e)
+def plan(out: DeviceArray, e, rows=None, cols=None) -> Plan:
+    """Prepare noalias(out) = e (or a reduction / scan into out); with rows= /
+    cols= as for assign_reduce, out = e with those reductions in one pass."""
+    return Plan(out, e, rows, cols)
 
 
 def execute_many(plans: Sequence[Plan], stream=None) -> None:

@@ -17,6 +17,7 @@
 
 #include <map>
 
+#include "assign_reduce_plan.hpp"
 #include "ew.hpp"
 #include "jit.hpp"
 #include "planner.hpp"
@@ -537,6 +538,118 @@ void reduce_any(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out,
 
 char* stream_workspace(size_t bytes, hipStream_t st, void*) { return static_cast<char*>(workspace(bytes, st)); }
 
+// out = e with reductions of its rows (axis 1) and / or columns (axis 0) in
+// the same pass (assign_reduce.hpp), planned: the kernel argument, the
+// kernel compiled at run time, and the column merge.
+struct ArcPlan {
+    AssignRC arg{};
+    const JitKernel* k = nullptr;
+    uint32_t bands = 0;
+    bool cols = false;
+    RedPlan comb{};     // the column merge (a split combine over the bands)
+    size_t ws = 0;      // workspace bytes: the column partials
+};
+
+// Plan the single pass; false, planning nothing, when it does not apply: a
+// 2-D row-major contiguous output of at least 2 x 2, a compact plan whose
+// value type is the output's and whose rows are whole work units (a plan that
+// merged the output to one dimension needs C % V == 0), rows of at most 512
+// units (8 a lane; fewer for elements under 4 bytes), reductions of the plain stored values (no mean, variance or square
+// root, no conversion of an element before it is folded) over exactly that
+// axis, and the kernel compiled at run time.
+bool plan_assign_rc(const xg_expr* e, const xg_tensor* out, const xg_reduce_desc* rr, const xg_tensor* ro,
+                    const xg_reduce_desc* cr, const xg_tensor* co, ArcPlan* ap) {
+    if (out->ndim != 2 || out->strides[1] != 1 || out->strides[0] != out->shape[1]) return false;
+    const int64_t R = out->shape[0], C = out->shape[1];
+    if (R < 2 || C < 2 || R >= (int64_t(1) << 31) || C >= (int64_t(1) << 31)) return false;
+    if (!jit_wanted(uint64_t(R) * uint64_t(C))) return false;
+    int V = 4;
+    EwPlan p = plan_assign_best(e, out, &V);
+    if (!p.compact || p.index64 || p.units == 0 || p.prog.result_dtype != out->dtype) return false;
+    uint32_t upr = 0;
+    if (p.rank == 2 && p.shape[0] == R && p.shape[1] == C) upr = p.nchunks;
+    else if (p.rank == 1 && C % V == 0) upr = uint32_t(C / V);
+    else return false;
+    // work units per lane: a wave holds a row (at most 8 units a lane, and
+    // the waves' column accumulators, 64 * UL * V of 16 bytes four times, in
+    // 128 KiB of LDS)
+    const uint32_t need = (upr + 63) / 64;
+    const int UL = need <= 1 ? 1 : (need <= 2 ? 2 : (need <= 4 ? 4 : 8));
+    if (need > 8 || UL * V > 32) return false;
+    // the reductions, planned over the output as a leaf (what xg_reduce of
+    // the stored values would run)
+    xg_node leaf{};
+    leaf.op = XG_OP_LEAF;
+    xg_tensor ol = *out;
+    xg_expr le{};
+    le.n_nodes = 1;
+    le.n_leaves = 1;
+    le.nodes = &leaf;
+    le.leaves = &ol;
+    const int cus = device_cus();
+    auto plain = [&](const xg_reduce_desc* d, int axis, RedPlan* q, const xg_tensor* o) {
+        if (d->n_axes != 1 || (d->axes[0] != axis && d->axes[0] != axis - 2)) return false;
+        if (d->op == XG_RED_MEAN || d->op == XG_RED_VARIANCE || d->op == XG_RED_STDDEV || d->op == XG_RED_MINMAX ||
+            d->op == XG_RED_NORM_L2)
+            return false;
+        *q = plan_reduce(&le, d, o, 4, cus, nullptr, nullptr);
+        return q->fin == FZ_NONE && q->vdt == out->dtype && q->prog.nops == 1 && q->prog.ops[0].op == MO_LOAD &&
+               q->n_out == uint32_t(axis == 1 ? R : C);
+    };
+    RedPlan rp{}, cp{};
+    if (rr && !plain(rr, 1, &rp, ro)) return false;
+    if (cr && !plain(cr, 0, &cp, co)) return false;
+    const int rak = rr ? rp.acc : -1, rop = rr ? rp.op : 0, cak = cr ? cp.acc : -1, cop = cr ? cp.op : 0;
+    const JitKernel* k = jit_assign_rc(p, V, rak, rop, cak, cop, UL);
+    if (!k) return false;
+    // bands of rows (a workgroup of eight waves each): about two per CU,
+    // column partials at most a quarter of the output's bytes
+    const uint64_t esz = uint64_t(dtype_size(out->dtype));
+    const uint64_t max_bands = std::max<uint64_t>(1, uint64_t(R) * esz / 64);
+    uint64_t rpb = std::max<uint64_t>((uint64_t(R) + 2 * cus - 1) / (2 * cus), (uint64_t(R) + max_bands - 1) / max_bands);
+    rpb = std::max<uint64_t>(rpb, 1);
+    ap->bands = uint32_t((uint64_t(R) + rpb - 1) / rpb);
+    ap->k = k;
+    ap->arg.plan = p;
+    FusedRC& f = ap->arg.f;
+    f.rows = uint32_t(R);
+    f.cols = uint32_t(C);
+    f.upr = upr;
+    f.rows_per_band = uint32_t(rpb);
+    if (rr) {
+        f.row_out = static_cast<char*>(ro->data) + ro->offset * dtype_size(ro->dtype);
+        f.row_stride = ro->ndim >= 1 ? ro->strides[0] : 0;
+        f.row_dtype = ro->dtype;
+        f.row_vdt = rp.vdt;
+        f.row_rdt = rp.rdt;
+        f.row_has_initial = rp.has_initial;
+        f.row_identity = rp.identity;
+        f.row_initial = rp.initial;
+    }
+    ap->cols = cr != nullptr;
+    if (cr) {
+        f.col_identity = cp.identity;
+        ap->ws = size_t(ap->bands) * size_t(C) * sizeof(Store);
+        RedPlan& q = ap->comb;
+        q = cp;
+        q.splits = int(ap->bands);
+        q.div_splits = make_fastdiv(ap->bands);
+        q.split_major = 1;
+        q.counters = nullptr;
+    }
+    return true;
+}
+
+// Enqueue a planned single pass; `ws` holds ap.ws bytes.
+void launch_assign_rc(ArcPlan& ap, void* ws, hipStream_t st) {
+    ap.arg.f.col_partials = static_cast<Store*>(ws);
+    check_hip(jit_launch(ap.k, ap.bands, 512, &ap.arg, sizeof(AssignRC), st), "fused assignment launch");
+    if (ap.cols) {
+        ap.comb.partials = ws;
+        check_hip(launch_reduce_combine(ap.comb, st), "column combine launch");
+    }
+}
+
 }  // namespace
 
 namespace {
@@ -641,6 +754,38 @@ xg_status xg_reduce_info(const xg_expr* e, const xg_reduce_desc* r, int32_t* dty
 
 xg_status xg_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, xg_stream_t stream) {
     return guard([&] { reduce_call(e, r, out, reinterpret_cast<hipStream_t>(stream), stream_workspace, nullptr, true); });
+}
+
+xg_status xg_assign_reduce2d(const xg_expr* e, const xg_tensor* out, const xg_reduce_desc* row_red,
+                             const xg_tensor* row_out, const xg_reduce_desc* col_red, const xg_tensor* col_out,
+                             xg_stream_t stream) {
+    return guard([&] {
+        if (!e || !out) raise(XG_INVALID_ARGUMENT, "null argument");
+        if (!row_red != !row_out || !col_red != !col_out)
+            raise(XG_INVALID_ARGUMENT, "a reduction needs both its descriptor and its output");
+        hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+        const char* used = nullptr;
+        ArcPlan ap;
+        if (plan_assign_rc(e, out, row_red, row_out, col_red, col_out, &ap)) {
+            launch_assign_rc(ap, ap.ws ? workspace(ap.ws, st) : nullptr, st);
+            note_kernel("assign_rc");
+            return;
+        }
+        // the same as three calls: the assignment, then each reduction of
+        // the stored output
+        assign_launch(e, out, st, &used);
+        xg_node leaf{};
+        leaf.op = XG_OP_LEAF;
+        xg_tensor ol = *out;
+        xg_expr le{};
+        le.n_nodes = 1;
+        le.n_leaves = 1;
+        le.nodes = &leaf;
+        le.leaves = &ol;
+        if (row_red) reduce_call(&le, row_red, row_out, st, stream_workspace, nullptr, true);
+        if (col_red) reduce_call(&le, col_red, col_out, st, stream_workspace, nullptr, true);
+        note_kernel("assign_then_reduce");
+    });
 }
 
 xg_status xg_scan_info(const xg_expr* e, const xg_scan_desc* s, int32_t* dtype, int32_t* ndim, int64_t* shape) {
@@ -867,8 +1012,9 @@ xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out,
 
 // ---------------------------------------------------------------- plans
 struct xg_plan_s {
-    int kind = 0;  // 0 assign, 1 reduce, 2 scan
+    int kind = 0;  // 0 assign, 1 reduce, 2 scan, 3 assign with row / column reductions
     int V = 4;
+    ArcPlan arc;
     xg::EwPlan ew;
     std::vector<std::pair<xg::EwPlan, int>> pieces;  // assign past 32-bit offsets: one plan per piece (ew unused)
     xg::RedPlan red;
@@ -956,11 +1102,35 @@ xg_status xg_plan_execute(xg_plan_t p, void* ws, xg_stream_t stream) {
                 p->red.counters = ws ? nullptr : reduce_counters(st);
                 check_hip(launch_reduce(p->red, p->V, st, &used), "reduction launch");
             }
+        } else if (p->kind == 3) {
+            launch_assign_rc(p->arc, p->ws ? (ws ? ws : workspace(p->ws, st)) : nullptr, st);
+            used = "assign_rc";
         } else {
             void* w = p->ws ? (ws ? ws : workspace(p->ws, st)) : nullptr;
             check_hip(launch_scan(p->scan, p->V, w, st, &used), "scan launch");
         }
         note_kernel(used);
+    });
+}
+
+xg_status xg_plan_assign_reduce2d(const xg_expr* e, const xg_tensor* out, const xg_reduce_desc* row_red,
+                                  const xg_tensor* row_out, const xg_reduce_desc* col_red, const xg_tensor* col_out,
+                                  xg_plan_t* plan) {
+    return guard([&] {
+        if (!plan || !e || !out) raise(XG_INVALID_ARGUMENT, "null argument");
+        if (!row_red != !row_out || !col_red != !col_out)
+            raise(XG_INVALID_ARGUMENT, "a reduction needs both its descriptor and its output");
+        auto* p = new xg_plan_s();
+        p->kind = 3;
+        try {
+            if (!plan_assign_rc(e, out, row_red, row_out, col_red, col_out, &p->arc))
+                raise(XG_UNSUPPORTED, "the single pass does not apply to this assignment (plan the three calls)");
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        p->ws = p->arc.ws;
+        *plan = p;
     });
 }
 
@@ -989,6 +1159,8 @@ xg_status xg_plan_execute_many(const xg_plan_t* plans, int64_t n, xg_stream_t st
                 p->red.partials = p->ws ? workspace(p->ws, st) : nullptr;
                 p->red.counters = reduce_counters(st);
                 check_hip(launch_reduce(p->red, p->V, st, &used), "reduction launch");
+            } else if (p->kind == 3) {
+                launch_assign_rc(p->arc, p->ws ? workspace(p->ws, st) : nullptr, st);
             } else if (p->kind == 2) {
                 check_hip(launch_scan(p->scan, p->V, p->ws ? workspace(p->ws, st) : nullptr, st, &used),
                           "scan launch");

@@ -308,6 +308,16 @@ const JitKernel* jit_reduce(const RedPlan& p, int V, int kind) {
     return get(src, expr);
 }
 
+const JitKernel* jit_assign_rc(const EwPlan& p, int V, int rak, int rop, int cak, int cop, int ul) {
+    if (!p.compact || p.rank < 1 || p.rank > 4) return nullptr;
+    const std::string src = "#include \"assign_reduce.hpp\"\n" + emit_ew_spec("JitSpec", p, V);
+    const std::string expr = "&xg::assign_rc_kernel<" + std::to_string(V) + ", " + std::to_string(p.prog.nregs) +
+                             ", JitSpec, " + std::to_string(p.rank) + ", " + std::to_string(rak) + ", " +
+                             std::to_string(rop) + ", " + std::to_string(cak) + ", " + std::to_string(cop) + ", " +
+                             std::to_string(ul) + ">";
+    return get(src, expr);
+}
+
 hipError_t jit_launch(const JitKernel* k, uint32_t grid, uint32_t block, const void* arg, size_t arg_bytes,
                       hipStream_t stream) {
     size_t bytes = arg_bytes;

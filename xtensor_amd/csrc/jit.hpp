@@ -44,6 +44,9 @@ int jit_set_thread_loaded_only(int on);
 const JitKernel* jit_ew(const EwPlan& p, int V);
 // compiled reduction kernel of the given kind for plan p at width V, or nullptr
 const JitKernel* jit_reduce(const RedPlan& p, int V, int kind);
+// assign_rc_kernel (assign_reduce.hpp) on plan p's program: RAK / CAK = the
+// row / column accumulator kinds (-1: none), UL work units per lane.
+const JitKernel* jit_assign_rc(const EwPlan& p, int V, int rak, int rop, int cak, int cop, int ul);
 
 // launch with the plan struct as the only kernel argument
 hipError_t jit_launch(const JitKernel* k, uint32_t grid, uint32_t block, const void* arg, size_t arg_bytes,

@@ -105,6 +105,8 @@ static hipError_t launch_combine(const RedPlan& p, hipStream_t stream) {
     return hipGetLastError();
 }
 
+hipError_t launch_reduce_combine(const RedPlan& p, hipStream_t stream) { return launch_combine(p, stream); }
+
 namespace {
 
 uint64_t plan_elements(const RedPlan& p) {

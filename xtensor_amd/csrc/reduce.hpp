@@ -116,6 +116,9 @@ RedFinish reduce_finish(const xg_expr* e, const xg_reduce_desc* r);
 
 #if !defined(__HIPCC_RTC__)  // host API (absent from run-time compiled sources)
 hipError_t launch_reduce(RedPlan& p, int V, hipStream_t stream, const char** used);
+// The split-combine launch alone: merges p.splits partials per output from
+// p.partials (laid out as p.split_major says) and finishes the outputs.
+hipError_t launch_reduce_combine(const RedPlan& p, hipStream_t stream);
 #endif
 
 // Arrival counters the cols fast kernel can merge its splits with: zeroed
