@@ -479,14 +479,24 @@ void xg_batch_destroy(xg_batch_t batch);
  *   XG_TPL_ROW_SUM:     out = sum(x, {1}), x [rows, cols]; float32/float64/int32
  *   XG_TPL_ROW_SOFTMAX: out = softmax of each row of x [rows, cols]; float32/float64
  *   XG_TPL_BITMIX:      out = (x * 3 + (x >> 2)) & 0xFFFF, x [n]; int32
- * All operands are contiguous row-major.  Results do not depend on what else
- * is in flight: the elementwise templates are exact functions of their
- * element, and a row's reduction order depends only on its length. */
+ *   XG_TPL_EXPR:        *out_tensor = *expr, any expression xg_assign takes
+ *                       (xt::noalias(out) = e); requests of one compiled
+ *                       program and rank are fused like the templates above
+ *   XG_TPL_REDUCE:      *out_tensor = reduce(*expr) with *reduce, any
+ *                       reduction xg_reduce takes (any axes); each runs alone
+ * The operands of the four fixed templates are contiguous row-major; EXPR and
+ * REDUCE take strided views.  `expr`, `out_tensor` and `reduce` are copied at
+ * submission (the device memory they point to must stay valid until the
+ * request completes).  Results do not depend on what else is in flight: the
+ * elementwise templates are exact functions of their element, and a
+ * reduction's order depends only on its own shape. */
 typedef enum xg_template {
     XG_TPL_AFFINE_CLIP = 0,
     XG_TPL_ROW_SUM = 1,
     XG_TPL_ROW_SOFTMAX = 2,
-    XG_TPL_BITMIX = 3
+    XG_TPL_BITMIX = 3,
+    XG_TPL_EXPR = 4,
+    XG_TPL_REDUCE = 5
 } xg_template;
 
 typedef struct xg_request {
@@ -500,6 +510,9 @@ typedef struct xg_request {
     uint64_t tag;            /* returned in the completion                    */
     int64_t arrival_ns;      /* xg_server_now_ns() time the request arrived;
                                 0 = the time of submission                     */
+    const xg_expr* expr;     /* EXPR / REDUCE: the expression                 */
+    const xg_tensor* out_tensor; /* EXPR / REDUCE: the output view            */
+    const xg_reduce_desc* reduce; /* REDUCE: the reduction                    */
 } xg_request;
 
 typedef struct xg_server_config {

@@ -245,3 +245,80 @@ def test_bad_requests_rejected():
         with pytest.raises(TypeError):
             srv.submit("bitmix", xf, xt.zeros((8,), "float32"))
         assert srv.stats()["rejected"] == 2
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_expression_and_reduction_requests(plain):
+    """Requests of any expression ("expr": noalias(out) = e, strided views and
+    broadcasts included) and any reduction ("reduce": any axes, variance
+    too), mixed with each other: each result is bit-identical to the same
+    expression evaluated alone through the single-call entry points, on the
+    fused server and on the plain per-request path."""
+    rng = np.random.default_rng(44)
+    a = xt.from_numpy(rng.normal(0, 1, (96, 130)).astype(np.float32))
+    b = xt.from_numpy(rng.normal(0, 1, (130,)).astype(np.float32))
+    big = xt.from_numpy(rng.normal(1, 1, (257, 512)))
+    ints = xt.from_numpy(rng.integers(-2**20, 2**20, (3000,), dtype=np.int32))
+    exprs = [
+        lambda: a * b + 1.5,
+        lambda: xt.clip(a.T * 2.0, -1, 1),
+        lambda: (ints * 3 + (ints >> 2)) & 0xFFFF,
+        lambda: xt.exp(-big * big),
+        lambda: big[:, xt.range_(0, 512, 2)] * 2.0,
+        lambda: ints * 7 - 1,
+        lambda: a * b + 1.5,  # the same program again (fused with the first)
+    ]
+    reds = [
+        lambda: xt.sum(big, [0]),
+        lambda: xt.amax(a.T, [1]),
+        lambda: xt.variance(big, [1]),
+        lambda: xt.sum(ints, None, dtype="int64"),
+        lambda: xt.mean(a * b, [0, 1]),
+    ]
+    reqs = []
+    for k in range(3):
+        for f in exprs:
+            reqs.append(("expr", f))
+        for f in reds:
+            reqs.append(("reduce", f))
+    refs, outs = [], []
+    for kind, f in reqs:
+        e = f()
+        ref = xt.eval(e)
+        refs.append(ref)
+        outs.append(xt.empty(ref.shape, ref.dtype))
+    sync()
+    cfg = dict(budget_bytes=1 << 30, streams=4, max_batch=8)
+    if plain:
+        cfg.update(streams=1, max_batch=1, plain=True)
+    with Server(**cfg) as srv:
+        srv.hold(True)
+        for k, (kind, f) in enumerate(reqs):
+            assert srv.submit_expr(outs[k], f(), tag=k) == 0
+        srv.hold(False)
+        srv.drain(60_000)
+        comps = srv.poll(1000)
+        st = srv.stats()
+    assert len(comps) == len(reqs) and all(c["status"] == 0 for c in comps), comps
+    if not plain:
+        assert st["launches"] < len(reqs)  # the programs repeated across requests were fused
+    sync()
+    for k, (o, r) in enumerate(zip(outs, refs)):
+        assert same_bits(o.to_numpy(), r.to_numpy()), (k, reqs[k][0])
+
+
+def test_expression_requests_rejected_or_failed():
+    """A malformed request is refused at submission; one whose output does not
+    match completes with an error status and leaves the server running."""
+    a = xt.from_numpy(np.ones((8, 8), np.float32))
+    with Server(budget_bytes=1 << 30, streams=2, max_batch=4) as srv:
+        with pytest.raises(Exception):
+            srv.submit_expr(xt.empty((8, 8), "float32"), a + xt.from_numpy(np.ones((3,), np.float32)), tag=1)
+        assert srv.submit_expr(xt.empty((5,), "float32"), xt.sum(a, [0]), tag=2) == 0
+        good = xt.empty((8, 8), "float32")
+        assert srv.submit_expr(good, a * 2, tag=3) == 0
+        srv.drain(30_000)
+        comps = {c["tag"]: c["status"] for c in srv.poll(10)}
+    assert comps[2] != 0 and comps[3] == 0
+    sync()
+    assert np.array_equal(good.to_numpy(), np.full((8, 8), 2, np.float32))

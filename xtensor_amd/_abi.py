@@ -117,7 +117,8 @@ class BatchTuning(C.Structure):
 
 class Request(C.Structure):
     _fields_ = [("tpl", C.c_int32), ("dtype", C.c_int32), ("shape", C.c_int64 * 4), ("in_", C.c_void_p),
-                ("params", C.c_void_p * 4), ("out", C.c_void_p), ("tag", C.c_uint64), ("arrival_ns", C.c_int64)]
+                ("params", C.c_void_p * 4), ("out", C.c_void_p), ("tag", C.c_uint64), ("arrival_ns", C.c_int64),
+                ("expr", C.c_void_p), ("out_tensor", C.c_void_p), ("reduce", C.c_void_p)]
 
 
 class ServerConfig(C.Structure):

@@ -56,7 +56,70 @@ struct Job {
     int V = 4;
     // row templates
     BItem item{};
+    // EXPR / REDUCE: owned copies of the caller's descriptors
+    std::vector<xg_node> nodes;
+    std::vector<xg_tensor> leaves;
+    std::vector<xg_scalar> consts;
+    xg_tensor out{};
+    xg_reduce_desc rd{};
+    int64_t out_elems = 0;
+
+    xg_expr expr() const {
+        xg_expr e{};
+        e.n_nodes = int32_t(nodes.size());
+        e.n_leaves = int32_t(leaves.size());
+        e.n_consts = int32_t(consts.size());
+        e.nodes = nodes.data();
+        e.leaves = leaves.data();
+        e.consts = consts.data();
+        return e;
+    }
 };
+
+bool expr_kind(int tpl) { return tpl == XG_TPL_EXPR || tpl == XG_TPL_REDUCE; }
+
+// bytes of device memory a strided view spans
+int64_t view_bytes(const xg_tensor& t) {
+    int64_t lo = 0, hi = 0;
+    for (int k = 0; k < t.ndim; ++k) {
+        if (t.shape[k] <= 0) return 0;
+        const int64_t ext = (t.shape[k] - 1) * t.strides[k];
+        (ext < 0 ? lo : hi) += ext;
+    }
+    return (hi - lo + 1) * dtype_size(t.dtype);
+}
+
+// EXPR / REDUCE: copy the caller's descriptors into the job and check them;
+// returns the device bytes the request holds (its views' spans)
+int64_t take_expr_request(Job& j) {
+    const xg_request& r = j.r;
+    if (!r.expr || !r.out_tensor || (r.tpl == XG_TPL_REDUCE && !r.reduce))
+        raise(XG_INVALID_ARGUMENT, "expression requests need expr, out_tensor (and reduce)");
+    const xg_expr& e = *r.expr;
+    if (e.n_nodes < 1 || e.n_leaves < 0 || e.n_consts < 0 || !e.nodes || (e.n_leaves && !e.leaves) ||
+        (e.n_consts && !e.consts))
+        raise(XG_INVALID_ARGUMENT, "malformed expression");
+    j.nodes.assign(e.nodes, e.nodes + e.n_nodes);
+    j.leaves.assign(e.leaves, e.leaves + e.n_leaves);
+    j.consts.assign(e.consts, e.consts + e.n_consts);
+    j.out = *r.out_tensor;
+    if (j.out.ndim < 0 || j.out.ndim > XG_MAX_RANK || j.out.dtype < 0 || j.out.dtype >= XG_NUM_DTYPES)
+        raise(XG_INVALID_ARGUMENT, "malformed output tensor");
+    if (r.tpl == XG_TPL_REDUCE) j.rd = *r.reduce;
+    j.r.expr = nullptr;  // the copies are the request's from here on
+    j.r.out_tensor = nullptr;
+    j.r.reduce = nullptr;
+    const xg_expr ce = j.expr();
+    const ExprInfo info = analyse(&ce);  // throws on a malformed or unbroadcastable tree
+    (void)info;
+    j.out_elems = 1;
+    for (int k = 0; k < j.out.ndim; ++k) j.out_elems *= j.out.shape[k];
+    int64_t bytes = view_bytes(j.out);
+    for (const xg_tensor& t : j.leaves) bytes += view_bytes(t);
+    return bytes;
+}
+
+char* slot_workspace(size_t bytes, hipStream_t st, void*) { return static_cast<char*>(workspace(bytes, st)); }
 
 // template, dtype, program, rank, V, solo (0: may be fused; else a unique id)
 using ClassKey = std::tuple<int, int, int, int, int, uint64_t>;
@@ -87,7 +150,7 @@ int64_t request_elems(const xg_request& r) {
     }
 }
 
-// validate; return the device bytes the request holds while admitted
+// validate a fixed template; return the device bytes the request holds while admitted
 int64_t check_request(const xg_request& r) {
     const int dt = r.dtype;
     const bool f = dt == XG_FLOAT32 || dt == XG_FLOAT64;
@@ -197,6 +260,23 @@ struct xg_server_s {
 
     ClassKey prepare(Job& j) {
         const xg_request& r = j.r;
+        if (r.tpl == XG_TPL_EXPR) {
+            const xg_expr e = j.expr();
+            j.ew = plan_assign_best(&e, &j.out, &j.V);
+            const int si = ew_spec_index(j.ew, j.V);
+            const uint64_t solo = (si < 0 || j.ew.units > kSoloUnits) ? j.seq + 1 : 0;
+            return ClassKey{r.tpl, j.out.dtype, si, j.ew.rank, j.V, solo};
+        }
+        if (r.tpl == XG_TPL_REDUCE) {  // axes and result type checked here; each runs alone
+            const xg_expr e = j.expr();
+            int dt = 0;
+            Shape rs;
+            reduce_result(&e, &j.rd, &dt, &rs);
+            if (int(rs.size()) != j.out.ndim) raise(XG_BROADCAST_ERROR, "reduction output rank mismatch");
+            for (size_t k = 0; k < rs.size(); ++k)
+                if (rs[k] != j.out.shape[k]) raise(XG_BROADCAST_ERROR, "reduction output shape mismatch");
+            return ClassKey{r.tpl, j.out.dtype, -1, 0, 0, j.seq + 1};
+        }
         if (r.tpl == XG_TPL_AFFINE_CLIP || r.tpl == XG_TPL_BITMIX) {
             Builder b;
             if (r.tpl == XG_TPL_AFFINE_CLIP) build_affine(b, r);
@@ -239,7 +319,16 @@ struct xg_server_s {
     // fused launch of one class group on slot s
     void launch_fused(Slot& s) {
         const xg_request& r0 = s.jobs[0].r;
-        if (r0.tpl == XG_TPL_AFFINE_CLIP || r0.tpl == XG_TPL_BITMIX) {
+        if (r0.tpl == XG_TPL_REDUCE) {
+            for (const Job& j : s.jobs) {
+                const xg_expr e = j.expr();
+                reduce_call(&e, &j.rd, &j.out, s.stream, slot_workspace, nullptr, true);
+            }
+            std::lock_guard<std::mutex> lk(mu);
+            ++launches;
+            return;
+        }
+        if (r0.tpl == XG_TPL_AFFINE_CLIP || r0.tpl == XG_TPL_BITMIX || r0.tpl == XG_TPL_EXPR) {
             std::vector<const EwPlan*> plans;
             std::vector<int> V;
             for (const Job& j : s.jobs) {
@@ -292,6 +381,13 @@ struct xg_server_s {
         int n = 0;
         for (const Job& j : s.jobs) {
             const xg_request& r = j.r;
+            if (expr_kind(r.tpl)) {
+                const xg_expr e = j.expr();
+                if (r.tpl == XG_TPL_EXPR) status_check(xg_assign(&e, &j.out, st), "assign");
+                else status_check(xg_reduce(&e, &j.rd, &j.out, st), "reduce");
+                ++n;
+                continue;
+            }
             if (request_elems(r) == 0 && r.tpl != XG_TPL_ROW_SUM) continue;
             if (r.tpl == XG_TPL_AFFINE_CLIP || r.tpl == XG_TPL_BITMIX) {
                 Builder b;
@@ -500,7 +596,7 @@ xg_status xg_server_submit(xg_server_t s, const xg_request* req) {
         Job j;
         j.r = *req;
         try {
-            j.bytes = check_request(j.r);
+            j.bytes = expr_kind(j.r.tpl) ? take_expr_request(j) : check_request(j.r);
         } catch (...) {
             std::lock_guard<std::mutex> lk(s->mu);
             ++s->rejected;

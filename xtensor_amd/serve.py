@@ -14,6 +14,10 @@ Templates (each is the xtensor expression of the same name):
     row_sum      out = sum(x, {1}), x [rows, cols]
     row_softmax  out = exp(x - amax(x, {1})) / sum(exp(x - amax(x, {1})), {1})
     bitmix       out = (x * 3 + (x >> 2)) & 0xFFFF, x int32 [n]
+    expr         out = e for any expression e (`submit_expr(out, e)`); requests
+                 of one compiled program and rank are fused
+    reduce       out = a reducer such as sum(x, [0]) (`submit_expr(out, r)`),
+                 any axes; each runs alone on a server stream
 """
 
 from __future__ import annotations
@@ -24,7 +28,7 @@ from typing import Sequence
 from . import _abi as A
 from .core import DeviceArray, _check
 
-TEMPLATES = {"affine_clip": 0, "row_sum": 1, "row_softmax": 2, "bitmix": 3}
+TEMPLATES = {"affine_clip": 0, "row_sum": 1, "row_softmax": 2, "bitmix": 3, "expr": 4, "reduce": 5}
 
 
 def now_ns() -> int:
@@ -56,6 +60,35 @@ class Server:
                tag: int = 0, arrival_ns: int = 0) -> int:
         """Queue a request; returns the library status (0 = queued, 8 = queue full)."""
         return self.submit_prepared(self.prepare(template, x, out, params, tag), arrival_ns)
+
+    def submit_expr(self, out: DeviceArray, e, tag: int = 0, arrival_ns: int = 0) -> int:
+        """Queue `noalias(out) = e` for any expression (template "expr"), or a
+        reduction into out when e is a reducer such as xt.sum(x, [0])
+        (template "reduce").  The descriptors are copied at submission; the
+        arrays stay referenced until the request completes."""
+        from .core import Reducer, _leaves, _require_device, _wrap, _eval_reducer, serialize
+        e = _wrap(e)
+        red = isinstance(e, Reducer)
+        body = e.expr if red else e
+        _require_device(_leaves(body) + [out])
+        s = serialize(body, reducer_fn=_eval_reducer)
+        t = out.to_abi()
+        r = A.Request()
+        r.tpl = TEMPLATES["reduce" if red else "expr"]
+        r.dtype = out.dtype_code
+        r.expr = C.addressof(s.expr)
+        r.out_tensor = C.addressof(t)
+        d = e.to_abi() if red else None
+        if red:
+            r.reduce = C.addressof(d)
+        r.tag = tag
+        r.arrival_ns = arrival_ns
+        st = A.lib().xg_server_submit(self.handle, C.byref(r))
+        if st == A.BUSY:
+            return st
+        _check(st)
+        self._keep[tag] = (out, e, _leaves(body))
+        return st
 
     def submit_prepared(self, prepared, arrival_ns: int = 0) -> int:
         """Queue a request built by prepare() (its checks already done), with
