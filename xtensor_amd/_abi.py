@@ -197,6 +197,7 @@ DEBUG_EXPORTS = {
     "xg_debug_fill_granules": (None, [C.c_int]),
     "xg_debug_jit": (None, [C.c_int]),
     "xg_debug_reduce_waves": (None, [C.c_int]),
+    "xg_debug_cols_split": (None, [C.c_int, C.c_int]),
     "xg_debug_scan_blocks_per_cu": (None, [C.c_int]),
     "xg_debug_jit_stats": (C.c_char_p, [C.POINTER(C.c_int64)]),
     "xg_debug_assign_kernel": (C.c_int, [_P(Expr), _P(Tensor), C.c_char_p, C.c_int32]),

@@ -1192,6 +1192,8 @@ void xg_debug_jit(int mode) { jit_set_mode(mode); }
 
 // Measurement aid: waves per CU the reduction work split aims at (0: default).
 void xg_debug_reduce_waves(int per_cu) { set_reduce_waves_per_cu(per_cu); }
+// Measurement aid: column reductions' workgroups per CU and partials budget multiplier.
+void xg_debug_cols_split(int blocks_per_cu, int partial_mult) { set_cols_split(blocks_per_cu, partial_mult); }
 
 // Measurement aid: workgroups per CU the tiled scans aim at (0: default).
 void xg_debug_scan_blocks_per_cu(int per_cu) { set_scan_blocks_per_cu(per_cu); }

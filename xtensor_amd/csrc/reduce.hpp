@@ -130,6 +130,9 @@ size_t reduce_workspace_bytes(const RedPlan& p);
 // Waves per CU the compiled reduction kernels' work split aims at (default
 // 16; measurement aid), for plans made after the call.
 void set_reduce_waves_per_cu(int w);
+// column fast kernel: workgroups per CU and the partials' byte budget
+// multiplier (measurement aid; 1, 1 = the defaults)
+void set_cols_split(int blocks_per_cu, int partial_mult);
 
 struct RSpecEntry;
 // compiled specialisation matching the plan, or nullptr

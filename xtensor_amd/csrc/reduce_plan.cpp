@@ -18,6 +18,14 @@ namespace xg {
 
 int g_reduce_waves_per_cu = 16;
 void set_reduce_waves_per_cu(int w) { g_reduce_waves_per_cu = w > 0 ? w : 16; }
+// column fast kernel: 512-thread workgroups per CU, and the split partials'
+// share of the input bytes, g_cols_partial_mult / 16
+int g_cols_blocks_per_cu = 1;
+int g_cols_partial_mult = 1;
+void set_cols_split(int blocks_per_cu, int partial_mult) {
+    g_cols_blocks_per_cu = blocks_per_cu > 0 ? blocks_per_cu : 1;
+    g_cols_partial_mult = partial_mult > 0 ? partial_mult : 1;
+}
 
 namespace {
 
@@ -604,11 +612,12 @@ RedPlan plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* 
             const uint64_t row_groups = 512 / tw;
             // blocks of 512 threads: one per CU at the default 16 waves per CU
             // (4.2 TB/s on a [8192, 4096] float32 column sum, 3.7 with two per CU)
-            const uint64_t target_blocks = std::max<uint64_t>(1, target_waves / 16);
+            const uint64_t target_blocks = std::max<uint64_t>(1, target_waves / 16 * uint64_t(g_cols_blocks_per_cu));
             uint64_t s = (target_blocks + tiles - 1) / tiles;
             s = std::min<uint64_t>(s, std::max<uint64_t>(1, rows / (row_groups * kU * 2)));
             // partial results stay within about 6% of the input bytes
-            s = std::min<uint64_t>(s, std::max<uint64_t>(1, uint64_t(best) / (uint64_t(n_out) * 16 * 16)));
+            s = std::min<uint64_t>(s, std::max<uint64_t>(1, uint64_t(best) * uint64_t(g_cols_partial_mult) /
+                                                                         (uint64_t(n_out) * 16 * 16)));
             p.f_group = int(tw);
             p.f_splits = int(std::min<uint64_t>(s, 65535));
             p.f_per_split = uint32_t((rows + p.f_splits - 1) / p.f_splits);
