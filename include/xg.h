@@ -380,6 +380,11 @@ typedef struct xg_plan_s* xg_plan_t;
 xg_status xg_plan_assign(const xg_expr* e, const xg_tensor* out, xg_plan_t* plan);
 xg_status xg_plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_tensor* out, xg_plan_t* plan);
 xg_status xg_plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, xg_plan_t* plan);
+/* Reductions and scans the one-launch plans cannot hold (variance, stddev and
+ * minmax, immediate evaluation, extents past 2^31 elements) are planned as
+ * their descriptors and run as xg_reduce / xg_scan on every execution: the
+ * host plans them each time, and a first execution may grow the stream's
+ * workspace, so run such a plan once before capturing it in a graph. */
 /* xg_assign_reduce2d prepared: XG_UNSUPPORTED when the single pass does not
  * apply to these descriptors (then plan the assignment and the reductions
  * apart). */

@@ -192,3 +192,32 @@ def test_scans_past_2_31():
     assert torch.equal(g[1], (j // 5 + 1).to(torch.int32))
     del got, g, x2, j
     torch.cuda.empty_cache()
+
+
+def test_prepared_plans_past_2_31():
+    """Prepared reduction and scan plans over a dimension past 2^31 elements
+    keep their descriptors and run as the direct calls (pieces) on each
+    execution: the same results as xg_reduce / xg_scan, twice in a row."""
+    import torch
+    n = (1 << 31) + (1 << 21) + 3
+    x = torch.randint(-100, 100, (n,), dtype=torch.int8, device="cuda")
+    X = wrap(x, (n,))
+    s = xt.empty((), "int64")
+    pl = xt.plan(s, xt.sum(X, None, dtype="int64"))
+    ref = int(x.sum(dtype=torch.int64).item())
+    for _ in range(2):
+        s.storage.zero_()
+        pl.run()
+        torch.cuda.synchronize()
+        assert int(s.to_numpy()) == ref
+    del pl, s
+    i = torch.arange(n, device="cuda", dtype=torch.int64)
+    y = (i % 3 == 0).to(torch.int32)
+    del x, X
+    out = xt.empty((n,), "int32")
+    ps = xt.plan(out, xt.Scan("cumsum", wrap(y, (n,)), 0))
+    ps.run()
+    torch.cuda.synchronize()
+    assert torch.equal(out.storage.view(torch.int32), (i // 3 + 1).to(torch.int32))
+    del ps, out, y, i
+    torch.cuda.empty_cache()

@@ -18,7 +18,7 @@ import pytest
 import xtensor_amd as xt
 from xtensor_amd import _abi as A
 
-from helpers import Dual, oracle_for, to_bf16_bits
+from helpers import Dual, oracle_for, same_bits, to_bf16_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -470,3 +470,24 @@ def test_immediate_column_major_and_float_results():
     got, _ = run(xt.variance(X.dev, [1], evaluation_strategy="immediate"))
     lazy, _ = run(xt.variance(X.dev, [1]))
     assert rel_err(got, lazy) <= 1e-12
+
+
+def test_prepared_multi_pass_and_immediate_reductions():
+    """variance / stddev / minmax and evaluation_strategy::immediate as
+    prepared plans: kept as descriptors, run as xg_reduce each execution,
+    bit-identical to the direct calls, also after the inputs change."""
+    import torch
+    rng = np.random.default_rng(71)
+    x = xt.from_numpy(rng.normal(0, 1, (300, 257)).astype(np.float32))
+    cases = [xt.variance(x, [1], ddof=1), xt.stddev(x, [0]), xt.minmax(x),
+             xt.sum(x * 2.0, [1], evaluation_strategy="immediate")]
+    for r in cases:
+        first = xt.eval(r)
+        out = xt.empty(first.shape, first.dtype)
+        pl = xt.plan(out, r)
+        for rep in range(2):
+            x.storage.view(torch.float32).mul_(1.5)
+            pl.run()
+            ref = xt.eval(r)
+            torch.cuda.synchronize()
+            assert same_bits(out.to_numpy(), ref.to_numpy()), (rep, r.kind)

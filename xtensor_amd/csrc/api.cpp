@@ -1012,9 +1012,41 @@ xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out,
 
 // ---------------------------------------------------------------- plans
 struct xg_plan_s {
-    int kind = 0;  // 0 assign, 1 reduce, 2 scan, 3 assign with row / column reductions
+    // 0 assign, 1 reduce, 2 scan, 3 assign with row / column reductions;
+    // 4 / 5: a reduction / scan the one-launch plans refuse (past 32-bit
+    // indexing, several passes, evaluated first), kept as its descriptors and
+    // run as xg_reduce / xg_scan at each execution
+    int kind = 0;
     int V = 4;
     ArcPlan arc;
+    std::vector<xg_node> d_nodes;
+    std::vector<xg_tensor> d_leaves;
+    std::vector<xg_scalar> d_consts;
+    xg_tensor d_out{};
+    xg_reduce_desc d_red{};
+    xg_scan_desc d_scan{};
+    void keep_call(const xg_expr* e, const xg_tensor* out) {
+        d_nodes.assign(e->nodes, e->nodes + e->n_nodes);
+        d_leaves.assign(e->leaves, e->leaves + e->n_leaves);
+        if (e->n_consts > 0) d_consts.assign(e->consts, e->consts + e->n_consts);
+        d_out = *out;
+    }
+    xg_expr call_expr() const {
+        xg_expr e{};
+        e.n_nodes = int32_t(d_nodes.size());
+        e.n_leaves = int32_t(d_leaves.size());
+        e.n_consts = int32_t(d_consts.size());
+        e.nodes = d_nodes.data();
+        e.leaves = d_leaves.data();
+        e.consts = d_consts.data();
+        return e;
+    }
+    void run_call(hipStream_t st) const {
+        const xg_expr e = call_expr();
+        const xg_status r = kind == 4 ? xg_reduce(&e, &d_red, &d_out, reinterpret_cast<xg_stream_t>(st))
+                                      : xg_scan(&e, &d_scan, &d_out, reinterpret_cast<xg_stream_t>(st));
+        if (r != XG_OK) raise(r, xg_last_error_message());
+    }
     xg::EwPlan ew;
     std::vector<std::pair<xg::EwPlan, int>> pieces;  // assign past 32-bit offsets: one plan per piece (ew unused)
     xg::RedPlan red;
@@ -1054,12 +1086,27 @@ xg_status xg_plan_reduce(const xg_expr* e, const xg_reduce_desc* r, const xg_ten
         auto* p = new xg_plan_s();
         try {
             p->kind = 1;
-            if (r && (r->op == XG_RED_VARIANCE || r->op == XG_RED_STDDEV || r->op == XG_RED_MINMAX))
-                raise(XG_UNSUPPORTED, "variance, stddev and minmax run several passes; use xg_reduce");
-            if (r && r->flags)
-                raise(XG_UNSUPPORTED, "immediate reductions evaluate their expression first; use xg_reduce");
-            p->red = plan_reduce_best(e, r, out, &p->V);
-            p->ws = reduce_workspace_bytes(p->red);
+            bool deferred = r && (r->op == XG_RED_VARIANCE || r->op == XG_RED_STDDEV || r->op == XG_RED_MINMAX ||
+                                  r->flags != 0);
+            if (!deferred) {
+                try {
+                    p->red = plan_reduce_best(e, r, out, &p->V);
+                    p->ws = reduce_workspace_bytes(p->red);
+                } catch (const Error& err) {
+                    if (err.code != XG_UNSUPPORTED) throw;
+                    deferred = true;  // past 32-bit indexing: xg_reduce cuts it into pieces
+                }
+            }
+            if (deferred) {
+                if (!r || !e || !out) raise(XG_INVALID_ARGUMENT, "null argument");
+                int dt = 0;
+                Shape rs;
+                reduce_result(e, r, &dt, &rs);  // checks the axes and types now
+                p->kind = 4;
+                p->keep_call(e, out);
+                p->d_red = *r;
+                p->ws = 0;
+            }
         } catch (...) {
             delete p;
             throw;
@@ -1076,8 +1123,16 @@ xg_status xg_plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor*
         try {
             p->kind = 2;
             p->V = 4;
-            p->scan = plan_scan(e, s, out, p->V, device_cus());
-            p->ws = scan_workspace_bytes(p->scan);
+            try {
+                p->scan = plan_scan(e, s, out, p->V, device_cus());
+                p->ws = scan_workspace_bytes(p->scan);
+            } catch (const Error& err) {
+                if (err.code != XG_UNSUPPORTED) throw;
+                p->kind = 5;  // past 32-bit indexing: xg_scan scans it in pieces
+                p->keep_call(e, out);
+                p->d_scan = *s;
+                p->ws = 0;
+            }
         } catch (...) {
             delete p;
             throw;
@@ -1105,6 +1160,9 @@ xg_status xg_plan_execute(xg_plan_t p, void* ws, xg_stream_t stream) {
         } else if (p->kind == 3) {
             launch_assign_rc(p->arc, p->ws ? (ws ? ws : workspace(p->ws, st)) : nullptr, st);
             used = "assign_rc";
+        } else if (p->kind >= 4) {
+            p->run_call(st);
+            used = p->kind == 4 ? "reduce_call" : "scan_call";
         } else {
             void* w = p->ws ? (ws ? ws : workspace(p->ws, st)) : nullptr;
             check_hip(launch_scan(p->scan, p->V, w, st, &used), "scan launch");
@@ -1161,6 +1219,8 @@ xg_status xg_plan_execute_many(const xg_plan_t* plans, int64_t n, xg_stream_t st
                 check_hip(launch_reduce(p->red, p->V, st, &used), "reduction launch");
             } else if (p->kind == 3) {
                 launch_assign_rc(p->arc, p->ws ? workspace(p->ws, st) : nullptr, st);
+            } else if (p->kind >= 4) {
+                p->run_call(st);
             } else if (p->kind == 2) {
                 check_hip(launch_scan(p->scan, p->V, p->ws ? workspace(p->ws, st) : nullptr, st, &used),
                           "scan launch");
