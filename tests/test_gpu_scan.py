@@ -217,7 +217,7 @@ def _tiled_input(rng, shape, dtype, kind):
 
 
 # (shape, tiled): the tiled kernels need rows of whole 16-byte vectors
-TILED_SHAPES = [((8192, 4096), True), ((1000, 300), True), ((77, 8), True), ((3, 1000, 132), True),
+TILED_SHAPES = [((8192, 4096), True), ((2048, 512), True), ((1000, 300), True), ((77, 8), True), ((3, 1000, 132), True),
                 ((130, 2, 260), True), ((77, 5), False), ((130, 2, 257), False)]
 
 
