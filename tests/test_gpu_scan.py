@@ -30,6 +30,12 @@ def sync():
     torch.cuda.synchronize()
 
 
+def tiled_form(name):
+    """The tiled scan a kernel name belongs to, whether it ran in three
+    launches or in one pass ("_1p")."""
+    return name[:-3] if name.endswith("_1p") else name
+
+
 def test_kat_one_d_short_to_int():
     """test_xaccumulator.cpp:23-34."""
     a = xt.from_numpy(np.array([1, 2, 3, 4], np.int16))
@@ -239,7 +245,7 @@ def test_tiled_column_scans(dtype, kind):
         X = xt.from_numpy(x)
         got = getattr(xt, kind)(X, axis)
         sync()
-        assert (xt.last_kernel() == "scan_tiles") == tiled, (shape, xt.last_kernel())
+        assert (tiled_form(xt.last_kernel()) == "scan_tiles") == tiled, (shape, xt.last_kernel())
         _check_scan(got.to_numpy(), _scan_ref(x, axis, kind), dtype, (kind, shape), kind, shape[axis])
 
 
@@ -254,7 +260,7 @@ def test_tiled_transposed_scans(dtype):
         X = xt.from_numpy(x)
         got = xt.cumsum(xt.transpose(X), 1)
         sync()
-        assert (xt.last_kernel() == "scan_tiles_trans") == tiled, (shape, xt.last_kernel())
+        assert (tiled_form(xt.last_kernel()) == "scan_tiles_trans") == tiled, (shape, xt.last_kernel())
         _check_scan(got.to_numpy(), _scan_ref(x.T, 1, "cumsum"), dtype, shape)
 
 
@@ -268,7 +274,7 @@ def test_tiled_scan_store_conversion_and_plans():
     out = xt.empty((600, 300), "int32")
     xt.plan(out, xt.Scan("cumsum", X.dev, 0)).run()
     sync()
-    assert xt.last_kernel() == "scan_tiles"
+    assert tiled_form(xt.last_kernel()) == "scan_tiles"
     ref = oracle_for(X).scan(X.dev, 0)
     assert np.array_equal(out.to_numpy(), ref.astype(np.int32))
     y = rng.integers(-50, 50, (600, 300)).astype(np.float32)
@@ -279,6 +285,80 @@ def test_tiled_scan_store_conversion_and_plans():
         p.run()
         sync()
         assert np.array_equal(o2.to_numpy(), np.cumsum(y.astype(np.float64), axis=0).astype(np.float32))
+
+
+# (shape, axis, transposed input): enough column tiles (outer x tiles >= 8)
+# for the one-pass form; partial column tiles, partial last chunks, a 3-d
+# outer dim and chains of up to 313 chunks (many look-back windows)
+ONE_PASS_CASES = [((8192, 4096), 0, False), ((300, 2100), 0, False), ((4, 200, 520), 1, False),
+                  ((20000, 2048), 0, False), ((65, 2304), 0, False),
+                  ((8192, 4096), 1, True), ((600, 2100), 1, True), ((100, 2052), 1, True)]
+
+
+def _one_pass_run(x, axis, trans, kind):
+    X = xt.from_numpy(x)
+    src = xt.transpose(X) if trans else X
+    got = getattr(xt, kind)(src, axis)
+    sync()
+    return got.to_numpy(), xt.last_kernel()
+
+
+@pytest.mark.parametrize("dtype", ["float32", "int32", "uint32"])
+@pytest.mark.parametrize("kind", ["cumsum", "cumprod"])
+def test_tiled_scans_one_pass(dtype, kind):
+    """Strided-axis scans of 32-bit accumulators over at least eight column
+    tiles run in one pass: every chunk of 64 rows is read once, its column
+    totals are published and each column looks back across the preceding
+    chunks for its carry.  Checked against the sequential reference (integers
+    exact, float32 sums rel 2e-7, products n ulps) and against the
+    three-launch form of the same scan (the knob that turns the one pass
+    off)."""
+    from xtensor_amd import _abi
+    lib = _abi.lib()
+    rng = np.random.default_rng(35)
+    for shape, axis, trans in ONE_PASS_CASES:
+        if kind == "cumprod" and (trans or np.prod(shape) > 3_000_000):
+            continue
+        x = _tiled_input(rng, shape, dtype, kind)
+        src = x.T if trans else x
+        got, name = _one_pass_run(x, axis, trans, kind)
+        want = "scan_tiles_trans_1p" if trans else "scan_tiles_1p"
+        assert name == want, (shape, name)
+        _check_scan(got, _scan_ref(src, axis, kind), dtype, (kind, shape), kind, src.shape[axis])
+        try:
+            lib.xg_debug_scan_single_pass(0)
+            three, name3 = _one_pass_run(x, axis, trans, kind)
+        finally:
+            lib.xg_debug_scan_single_pass(1)
+        assert name3 == want[:-3], (shape, name3)
+        if np.dtype(dtype).kind == "f":
+            scale = np.maximum(np.abs(three.astype(np.float64)), 1.0)
+            bar = 2e-7 if kind == "cumsum" else 2 * src.shape[axis] * 2.0 ** -24
+            assert np.max(np.abs(got.astype(np.float64) - three) / scale) <= bar, (kind, shape)
+        else:
+            assert np.array_equal(got, three), (kind, shape)
+
+
+def test_tiled_scan_one_pass_repeat_and_plan():
+    """The one pass clears its ticket counter and publications before every
+    launch: a prepared plan replayed on new inputs, and the same plan after a
+    scan of another shape reused the allocator's memory, stay exact (integer
+    data)."""
+    rng = np.random.default_rng(36)
+    out = xt.empty((3000, 2560), "int32")
+    xs = [rng.integers(-1000, 1000, (3000, 2560)).astype(np.int32) for _ in range(3)]
+    X = xt.empty((3000, 2560), "int32")
+    p = xt.plan(out, xt.Scan("cumsum", X, 0))
+    for x in xs:
+        xt.assign(X, xt.from_numpy(x))
+        for _ in range(2):
+            p.run()
+            sync()
+            assert xt.last_kernel() == "scan_tiles_1p"
+            assert np.array_equal(out.to_numpy(), np.cumsum(x, axis=0, dtype=np.int32))
+        small = xt.cumsum(xt.from_numpy(x[:700, :2048]), 0)
+        sync()
+        assert np.array_equal(small.to_numpy(), np.cumsum(x[:700, :2048], axis=0, dtype=np.int32))
 
 
 @pytest.mark.parametrize("dtype", ["float32", "float64", "int32", "int64"])

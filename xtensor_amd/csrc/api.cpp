@@ -1257,6 +1257,8 @@ void xg_debug_cols_split(int blocks_per_cu, int partial_mult) { set_cols_split(b
 
 // Measurement aid: workgroups per CU the tiled scans aim at (0: default).
 void xg_debug_scan_blocks_per_cu(int per_cu) { set_scan_blocks_per_cu(per_cu); }
+// Measurement aid: 0 keeps strided-axis scans on the three-launch tiled form.
+void xg_debug_scan_single_pass(int on) { set_scan_single_pass(on); }
 
 // Counters of the run-time compiler: compiles, disk-cache hits, in-process
 // hits, failures (out[0..3]); returns the last failure message.

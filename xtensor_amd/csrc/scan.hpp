@@ -35,7 +35,18 @@ struct TileScan {
     uint64_t identity;
     int32_t out_dtype, rdt;
     int32_t nt_in, nt_out;
+    // one pass (scan_tiled_1p.hpp): chunks of kTileLbRows rows, look-back
+    // across the chunks; the workspace holds the ticket counter and the
+    // publications
+    int32_t single;
+    uint32_t* ctrl;       // [0] ticket counter, [1] look-back timeouts
+    uint64_t* words;      // [outer][nchunks][ncols] packed value + flag
 };
+
+// Rows of one chunk of the one-pass tiled scan (held in registers whole),
+// and the fewest column tiles (outer x tiles) it takes.
+constexpr uint32_t kTileLbRows = 64;
+constexpr uint64_t kTileLbMinTiles = 8;
 
 // Segments of 64 lanes x 16 bytes per wave in one chunk of the look-back scan
 // (a chunk is 4 waves x kLbSegments segments).
@@ -90,6 +101,9 @@ struct ScanPlan {
 // Measurement aid: workgroups per compute unit the tiled scans' chunking
 // aims at, for plans made after the call (0: the default, 8).
 void set_scan_blocks_per_cu(int n);
+// Measurement aid: whether the tiled scans may take the one-pass form
+// (default 1), for plans made after the call.
+void set_scan_single_pass(int on);
 
 // The direct rows kernels apply: rank <= 2 rows layout with 32-bit offsets,
 // a program that is one load of a leaf of the accumulation input type, the

@@ -326,6 +326,7 @@ __global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p)
 
 #include "scan_tiled.hpp"
 #include "scan_lookback.hpp"
+#include "scan_tiled_1p.hpp"
 
 namespace xg {
 
@@ -337,12 +338,19 @@ using SFn = void (*)(const ScanPlan);
 using TFn = void (*)(const TileScan);
 using LFn = void (*)(const LbScan);
 
+// the one-pass tiled kernel where the accumulator publishes packed (else null)
+template <int AK, int OP, bool NATIVE, bool TRANS> constexpr TFn tile_1p_fn() {
+    if constexpr (LbPacked<AK>::value && (NATIVE || !TRANS)) return scan_tiles_1p_kernel<AK, OP, NATIVE, TRANS>;
+    else return nullptr;
+}
+
 struct ScanKernels {
     SFn rows32, rows64, cols32, cols64;
     SFn direct[2][2];   // [native][waves per row: 1, 4]
     uint32_t wave_tile;  // elements of one wave's tile in the direct kernels
     TFn tile_totals, tile_carry;
     TFn tiles[2][2];     // [native][transposed output]; the transposed one is native only
+    TFn tiles1p[2][2];   // the one-pass form, likewise (32-bit accumulators only)
     LFn lookback[2];     // [native]
 };
 
@@ -358,6 +366,8 @@ template <int AK, int OP> ScanKernels scan_kernels() {
                        scan_tiles_carry_kernel<AK, OP>,
                        {{scan_tiles_kernel<AK, OP, false, false>, nullptr},
                         {scan_tiles_kernel<AK, OP, true, false>, scan_tiles_kernel<AK, OP, true, true>}},
+                       {{tile_1p_fn<AK, OP, false, false>(), nullptr},
+                        {tile_1p_fn<AK, OP, true, false>(), tile_1p_fn<AK, OP, true, true>()}},
                        {scan_lookback_kernel<AK, OP, false>, scan_lookback_kernel<AK, OP, true>}};
 }
 

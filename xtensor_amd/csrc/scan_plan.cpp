@@ -66,6 +66,7 @@ void scan_result(const xg_expr* e, const xg_scan_desc* s, int* dtype, Shape* sha
 namespace {
 
 int g_tile_blocks_per_cu = 8;  // workgroups per compute unit the tiled scans aim at
+int g_tile_single_pass = 1;    // the one-pass tiled form may apply
 
 // element type the direct and tiled kernels read, per accumulator kind
 int scan_in_dtype(int acc) {
@@ -136,6 +137,18 @@ void plan_tiled(ScanPlan* pp, int cus) {
     t.chunk_rows = uint32_t((rows + sub - 1) / sub * sub);
     t.nchunks = uint32_t((uint64_t(t.len) + t.chunk_rows - 1) / t.chunk_rows);
     if (per_chunk * t.nchunks >= (uint64_t(1) << 31)) return;
+    // One pass with a look-back per column: 32-bit accumulators (published
+    // packed with their flag), at least two chunks, and enough column tiles
+    // side by side (the front of published prefixes advances a window of
+    // chunks per look-back round trip in each tile; measured on MI355X).
+    const bool packed = p.acc == AK_F32C || p.acc == AK_F32 || p.acc == AK_I32 || p.acc == AK_U32;
+    const uint64_t chunks1 = (uint64_t(t.len) + kTileLbRows - 1) / kTileLbRows;
+    if (g_tile_single_pass && packed && per_chunk >= kTileLbMinTiles && chunks1 >= 2 &&
+        per_chunk * chunks1 < (uint64_t(1) << 31)) {
+        t.single = 1;
+        t.chunk_rows = kTileLbRows;
+        t.nchunks = uint32_t(chunks1);
+    }
     t.in = l.ptr;
     t.out = p.out_ptr;
     t.identity = p.identity;
@@ -185,6 +198,7 @@ size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 }  // namespace
 
 void set_scan_blocks_per_cu(int n) { g_tile_blocks_per_cu = n > 0 ? n : 8; }
+void set_scan_single_pass(int on) { g_tile_single_pass = on ? 1 : 0; }
 
 bool scan_direct_ok(const ScanPlan& p) {
     // rows are addressed with 64-bit offsets and elements within a row with
@@ -209,6 +223,7 @@ size_t scan_workspace_bytes(const ScanPlan& p) {
         return 256 + align256(n * 4) + 2 * align256(n * sizeof(Store));
     }
     if (p.tiled == ST_NONE || p.tile.nchunks <= 1) return 0;
+    if (p.tile.single) return 256 + align256(size_t(p.tile.outer) * p.tile.nchunks * p.tile.ncols * 8);
     return size_t(p.tile.outer) * p.tile.nchunks * p.tile.ncols * sizeof(Store);
 }
 
