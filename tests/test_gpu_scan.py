@@ -287,11 +287,12 @@ def test_tiled_scan_store_conversion_and_plans():
         assert np.array_equal(o2.to_numpy(), np.cumsum(y.astype(np.float64), axis=0).astype(np.float32))
 
 
-# (shape, axis, transposed input): enough column tiles (outer x tiles >= 8)
+# (shape, axis, transposed input): enough column tiles (outer x tiles >= 8;
+# 9, 10, 12 and 16 of them: one, two, four and eight ticket counters)
 # for the one-pass form; partial column tiles, partial last chunks, a 3-d
 # outer dim and chains of up to 313 chunks (many look-back windows)
 ONE_PASS_CASES = [((8192, 4096), 0, False), ((300, 2100), 0, False), ((4, 200, 520), 1, False),
-                  ((20000, 2048), 0, False), ((65, 2304), 0, False),
+                  ((20000, 2048), 0, False), ((65, 2304), 0, False), ((130, 2560), 0, False),
                   ((8192, 4096), 1, True), ((600, 2100), 1, True), ((100, 2052), 1, True)]
 
 

@@ -12,12 +12,12 @@
 
 namespace xg {
 
-// clears the one-pass tiled scan's ticket counter, timeout count and
+// clears the one-pass tiled scan's ticket counters, timeout count and
 // publications (a kernel, not a memset, so that the pair can be captured in a
 // HIP graph); `n2` = publications / 2
 __global__ __launch_bounds__(256) void scan_tiles_1p_init_kernel(uint32_t* ctrl, uint4* words, uint64_t n2) {
     const uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x;
-    if (i < 2) ctrl[i] = 0;
+    if (i <= kTileMaxClasses) ctrl[i * kTileCtrlStride] = 0;  // the counters, then the timeouts
     if (i < n2) words[i] = make_uint4(0, 0, 0, 0);
 }
 
@@ -40,11 +40,13 @@ hipError_t launch_scan(const ScanPlan& p, int V, void* ws, hipStream_t stream, c
         if (t.single) {  // one pass: chunks of kTileLbRows rows and a look-back per column
             TFn one = k.tiles1p[p.tile_native ? 1 : 0][trans ? 1 : 0];
             if (!one || t.chunk_rows != kTileLbRows) return hipErrorInvalidValue;
-            t.ctrl = static_cast<uint32_t*>(ws);
-            t.words = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256);
             const uint64_t total = uint64_t(t.outer) * t.nchunks * t.ntiles;
+            if (t.classes < 1 || t.classes > kTileMaxClasses || (uint64_t(t.outer) * t.ntiles) % t.classes)
+                return hipErrorInvalidValue;
+            t.ctrl = static_cast<uint32_t*>(ws);
+            t.words = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + kTileCtrlBytes);
             const uint64_t n2 = uint64_t(t.outer) * t.nchunks * t.ncols / 2;  // ncols: a multiple of 256
-            hipLaunchKernelGGL(scan_tiles_1p_init_kernel, dim3(uint32_t((n2 + 255) / 256)), block, 0, stream, t.ctrl,
+            hipLaunchKernelGGL(scan_tiles_1p_init_kernel, dim3(uint32_t((std::max<uint64_t>(n2, 16) + 255) / 256)), block, 0, stream, t.ctrl,
                                reinterpret_cast<uint4*>(t.words), n2);
             hipLaunchKernelGGL(one, dim3(uint32_t(total)), block, 0, stream, t);
             if (used) *used = trans ? "scan_tiles_trans_1p" : "scan_tiles_1p";

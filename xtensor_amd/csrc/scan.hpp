@@ -39,7 +39,8 @@ struct TileScan {
     // across the chunks; the workspace holds the ticket counter and the
     // publications
     int32_t single;
-    uint32_t* ctrl;       // [0] ticket counter, [1] look-back timeouts
+    uint32_t classes;     // ticket counters: chain k (= o * ntiles + tile) draws from counter k % classes
+    uint32_t* ctrl;       // counter k at ctrl[k * kTileCtrlStride]; timeouts at ctrl[kTileTimeouts]
     uint64_t* words;      // [outer][nchunks][ncols] packed value + flag
 };
 
@@ -47,6 +48,13 @@ struct TileScan {
 // and the fewest column tiles (outer x tiles) it takes.
 constexpr uint32_t kTileLbRows = 64;
 constexpr uint64_t kTileLbMinTiles = 8;
+// its ticket counters (at most kTileMaxClasses, 256 bytes apart) and timeout
+// count at the head of the workspace
+constexpr uint32_t kTileMaxClasses = 16;
+constexpr uint32_t kTileCtrlStride = 64;  // words
+constexpr uint32_t kTileTimeouts = kTileMaxClasses * kTileCtrlStride;
+constexpr size_t kTileCtrlBytes = 8192;
+static_assert((kTileTimeouts + 1) * 4 <= kTileCtrlBytes, "the counters fit the workspace head");
 
 // Segments of 64 lanes x 16 bytes per wave in one chunk of the look-back scan
 // (a chunk is 4 waves x kLbSegments segments).

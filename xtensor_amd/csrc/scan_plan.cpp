@@ -148,6 +148,8 @@ void plan_tiled(ScanPlan* pp, int cus) {
         t.single = 1;
         t.chunk_rows = kTileLbRows;
         t.nchunks = uint32_t(chunks1);
+        t.classes = 1;  // the largest power of two up to kTileMaxClasses dividing the chains
+        while (t.classes < kTileMaxClasses && per_chunk % (2 * t.classes) == 0) t.classes *= 2;
     }
     t.in = l.ptr;
     t.out = p.out_ptr;
@@ -223,7 +225,7 @@ size_t scan_workspace_bytes(const ScanPlan& p) {
         return 256 + align256(n * 4) + 2 * align256(n * sizeof(Store));
     }
     if (p.tiled == ST_NONE || p.tile.nchunks <= 1) return 0;
-    if (p.tile.single) return 256 + align256(size_t(p.tile.outer) * p.tile.nchunks * p.tile.ncols * 8);
+    if (p.tile.single) return kTileCtrlBytes + align256(size_t(p.tile.outer) * p.tile.nchunks * p.tile.ncols * 8);
     return size_t(p.tile.outer) * p.tile.nchunks * p.tile.ncols * sizeof(Store);
 }
 

@@ -6,9 +6,14 @@
 // launches of scan_tiled.hpp read the input twice; here a workgroup holds its
 // whole chunk (64 rows of one 256-column tile, 16 values per lane) in
 // registers, so the input is read once:
-//   * a ticket from an atomic counter orders the workgroups (column tile
-//     fastest), so the chunk before this one in the same tile was taken
-//     earlier and is resident or finished when this one waits for it;
+//   * a ticket from an atomic counter orders the workgroups (chunk-major), so
+//     the chunk before this one in the same column tile was taken earlier and
+//     is resident or finished when this one waits for it.  The column tiles
+//     (chains) are split into `classes` groups with a counter each: the
+//     workgroup with blockIdx b draws from counter b % classes and takes a
+//     chunk of that group's chains, so the ordering holds inside each group
+//     and one counter serves 1/classes of the workgroups (a single counter
+//     for 2048 workgroups measured ~11 us of 67 on MI355X);
 //   * the chunk's column totals meet in LDS and every thread publishes its
 //     column's total as an aggregate;
 //   * one wave looks back along column 0 over the 64 preceding chunks at
@@ -23,7 +28,7 @@
 // accumulators: float32 sums with compensation, float32 products, 32-bit
 // integers), written and read with single device-scope atomic accesses: no
 // fence.  Every wait is bounded (kSpinLimit polls; then the chunk counts as a
-// prefix of the identity and ctrl[1] counts the timeout: a wrong result,
+// prefix of the identity and the workspace counts the timeout: a wrong result,
 // never a hang).  The look-back merges in an order that depends on timing, so
 // float sums can differ in the last bit between runs (integers are exact).
 //
@@ -82,10 +87,10 @@ __device__ __forceinline__ uint32_t tile_lead_depth(uint64_t* col0, uint32_t str
 // column's publications follow column 0's closely: a slot not yet published,
 // or slot d still holding the aggregate, is polled again).  Bounded waits:
 // after kSpinLimit polls the slot counts as a prefix of the identity and
-// ctrl[1] counts the timeout.
+// `timeouts` counts it.
 template <int AK, int OP>
 __device__ __forceinline__ Acc<AK, OP> tile_follow(uint64_t* col, uint32_t stride, uint32_t c, uint32_t d,
-                                                   const Acc<AK, OP>& id, uint32_t* ctrl) {
+                                                   const Acc<AK, OP>& id, uint32_t* timeouts) {
     using A = Acc<AK, OP>;
     constexpr int B = kTileLbBatch;
     A excl = id;
@@ -105,7 +110,7 @@ __device__ __forceinline__ Acc<AK, OP> tile_follow(uint64_t* col, uint32_t strid
             int spins = 0;
             while (f == 0 || (m == d && f != kLbPrefix)) {
                 if (++spins >= kSpinLimit) {
-                    atomicAdd(ctrl + 1, 1u);
+                    atomicAdd(timeouts, 1u);
                     return excl;
                 }
                 __builtin_amdgcn_s_sleep(1);
@@ -128,9 +133,11 @@ __global__ __launch_bounds__(256) void scan_tiles_1p_kernel(const TileScan p) {
     constexpr int V = int(16 / sizeof(T));
     constexpr int COLS = 64 * V;
     static_assert(COLS == 256, "one column of the tile per thread in the look-back");
-    constexpr int RW = TRANS ? 8 : 16;          // rows per wave in a sub-tile
+    constexpr int CH = int(kTileLbRows);
+    constexpr int RW = TRANS ? (CH / 4 < 8 ? CH / 4 : 8) : CH / 4;  // rows per wave in a sub-tile
     constexpr uint32_t SUB = 4u * RW;           // rows of a sub-tile
     constexpr int NS = int(kTileLbRows / SUB);  // sub-tiles of a chunk, all held in registers
+    static_assert(NS * int(SUB) == CH, "a chunk is whole sub-tiles");
     constexpr int PITCH = int(SUB) + 1;
     static_assert(!TRANS || NATIVE, "the transposed scan stores the input's type");
     union Shared {
@@ -142,14 +149,17 @@ __global__ __launch_bounds__(256) void scan_tiles_1p_kernel(const TileScan p) {
     __shared__ uint32_t s_ticket, s_depth;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    if (threadIdx.x == 0) s_ticket = atomicAdd(p.ctrl, 1u);
+    const uint32_t cls = blockIdx.x % p.classes;
+    if (threadIdx.x == 0) s_ticket = atomicAdd(p.ctrl + cls * kTileCtrlStride, 1u);
     __syncthreads();
-    uint32_t b = s_ticket;
-    if (b >= p.outer * p.nchunks * p.ntiles) return;  // uniform (the grid holds exactly that many)
-    const uint32_t tile = b % p.ntiles;
-    b /= p.ntiles;
-    const uint32_t chunk = b % p.nchunks;
-    const uint32_t o = b / p.nchunks;
+    // the group's chains, chunk-major: ticket i -> chunk i / per, chain cls + classes * (i % per)
+    const uint32_t per = p.outer * p.ntiles / p.classes;
+    const uint32_t i = s_ticket;
+    if (i >= per * p.nchunks) return;  // uniform (the grid holds exactly classes * per * nchunks)
+    const uint32_t chunk = i / per;
+    const uint32_t chain = cls + p.classes * (i - chunk * per);
+    const uint32_t tile = chain % p.ntiles;
+    const uint32_t o = chain / p.ntiles;
     const uint32_t col = tile * uint32_t(COLS) + uint32_t(lane * V);
     const int nv = col < p.inner ? min(V, int(p.inner - col)) : 0;
     const uint32_t r0 = chunk * kTileLbRows;
@@ -222,7 +232,7 @@ __global__ __launch_bounds__(256) void scan_tiles_1p_kernel(const TileScan p) {
                 if (lane == 0) s_depth = d;
             }
             __syncthreads();
-            excl = tile_follow<AK, OP>(wtile + c, p.ncols, chunk, s_depth, id, p.ctrl);
+            excl = tile_follow<AK, OP>(wtile + c, p.ncols, chunk, s_depth, id, p.ctrl + kTileTimeouts);
             A incl = excl;
             incl.merge(agg);
             lb_store(mine, lb_pack(incl, kLbPrefix));
