@@ -362,8 +362,22 @@ def test_tiled_scan_one_pass_repeat_and_plan():
         assert np.array_equal(small.to_numpy(), np.cumsum(x[:700, :2048], axis=0, dtype=np.int32))
 
 
+@pytest.fixture(params=["scan_chunks3", "scan_lookback"])
+def long_row_form(request):
+    """Few long rows run in chunks of 4 x 16 x 64 x 16 bytes, either as the
+    three-launch reduce-then-scan (the default) or as the one-pass look-back
+    (the knob that selects it)."""
+    from xtensor_amd import _abi
+    lib = _abi.lib()
+    lib.xg_debug_scan_flat_single_pass(1 if request.param == "scan_lookback" else 0)
+    try:
+        yield request.param
+    finally:
+        lib.xg_debug_scan_flat_single_pass(0)
+
+
 @pytest.mark.parametrize("dtype", ["float32", "float64", "int32", "int64"])
-def test_lookback_long_rows(dtype):
+def test_lookback_long_rows(dtype, long_row_form):
     """Few rows much longer than one chunk (16384 elements of 4-byte types,
     8192 of 8-byte types) run the single-pass look-back scan: the flattened
     cumsum of a large array, and cumsum along axis 1 of a few long rows.
@@ -379,17 +393,30 @@ def test_lookback_long_rows(dtype):
         X = xt.from_numpy(x)
         got = xt.cumsum(X)
         sync()
-        assert xt.last_kernel() == "scan_lookback", (n, xt.last_kernel())
+        assert xt.last_kernel() == long_row_form, (n, xt.last_kernel())
         _check_scan(got.to_numpy(), _scan_ref(x, 0, "cumsum"), dtype, n)
     x = _tiled_input(rng, (3, 1_000_004), dtype, "cumsum")  # rows of whole 16-byte vectors
     X = xt.from_numpy(x)
     got = xt.cumsum(X, 1)
     sync()
-    assert xt.last_kernel() == "scan_lookback"
+    assert xt.last_kernel() == long_row_form
     _check_scan(got.to_numpy(), _scan_ref(x, 1, "cumsum"), dtype, "3 rows")
 
 
-def test_lookback_cumprod_repeat_and_expression():
+def test_chunked_scan_past_summed_carries():
+    """A row of more than 4096 chunks of 16384 elements: the reduce-then-scan
+    form merges the chunk totals into carries in a launch of its own instead
+    of in every chunk's workgroup.  int32, exact (values in -3..3)."""
+    rng = np.random.default_rng(43)
+    n = 4097 * 16384 + 5
+    x = rng.integers(-3, 4, n).astype(np.int32)
+    got = xt.cumsum(xt.from_numpy(x))
+    sync()
+    assert xt.last_kernel() == "scan_chunks3"
+    assert np.array_equal(got.to_numpy(), np.cumsum(x, dtype=np.int32))
+
+
+def test_lookback_cumprod_repeat_and_expression(long_row_form):
     """cumprod of +-1 integers and of values near 1 through the look-back
     kernel; the same scan twice in a row (the chunk flags are cleared before
     every launch); the flattened scan of an expression (evaluated into a
@@ -399,7 +426,7 @@ def test_lookback_cumprod_repeat_and_expression():
     P = xt.from_numpy(p)
     for _ in range(2):
         got = xt.cumprod(P).to_numpy()
-        assert xt.last_kernel() == "scan_lookback"
+        assert xt.last_kernel() == long_row_form
         assert np.array_equal(got, np.cumprod(p))
     f = rng.uniform(0.9999, 1.0001, 500_000).astype(np.float64)
     got = xt.cumprod(xt.from_numpy(f)).to_numpy()
@@ -409,6 +436,6 @@ def test_lookback_cumprod_repeat_and_expression():
     A_ = Dual.of(a)
     e = A_.dev * 2.0 + 1.0  # float64 values (double literals)
     got = xt.cumsum(e).to_numpy()
-    assert xt.last_kernel() == "scan_lookback"
+    assert xt.last_kernel() == long_row_form
     ref = _scan_ref((a.astype(np.float64) * 2.0 + 1.0).ravel(), 0, "cumsum")
     assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)) <= 1e-14

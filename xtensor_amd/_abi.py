@@ -200,6 +200,7 @@ DEBUG_EXPORTS = {
     "xg_debug_cols_split": (None, [C.c_int, C.c_int]),
     "xg_debug_scan_blocks_per_cu": (None, [C.c_int]),
     "xg_debug_scan_single_pass": (None, [C.c_int]),
+    "xg_debug_scan_flat_single_pass": (None, [C.c_int]),
     "xg_debug_jit_stats": (C.c_char_p, [C.POINTER(C.c_int64)]),
     "xg_debug_assign_kernel": (C.c_int, [_P(Expr), _P(Tensor), C.c_char_p, C.c_int32]),
     "xg_debug_assign_pieces": (C.c_int, [_P(Expr), _P(Tensor), _P(C.c_int64), _P(C.c_int32), _P(C.c_int64)]),

@@ -1259,6 +1259,8 @@ void xg_debug_cols_split(int blocks_per_cu, int partial_mult) { set_cols_split(b
 void xg_debug_scan_blocks_per_cu(int per_cu) { set_scan_blocks_per_cu(per_cu); }
 // Measurement aid: 0 keeps strided-axis scans on the three-launch tiled form.
 void xg_debug_scan_single_pass(int on) { set_scan_single_pass(on); }
+// Measurement aid: 1 runs few long rows as the one-pass look-back scan (default 0: three launches).
+void xg_debug_scan_flat_single_pass(int on) { set_scan_flat_single_pass(on); }
 
 // Counters of the run-time compiler: compiles, disk-cache hits, in-process
 // hits, failures (out[0..3]); returns the last failure message.

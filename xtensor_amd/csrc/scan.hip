@@ -61,6 +61,18 @@ hipError_t launch_scan(const ScanPlan& p, int V, void* ws, hipStream_t stream, c
         if (used) *used = trans ? "scan_tiles_trans" : "scan_tiles";
         return hipGetLastError();
     }
+    if (p.lookback && p.lb_three) {  // reduce-then-scan over the look-back's chunks
+        LbScan l = p.lb;
+        if (!ws) return hipErrorInvalidValue;
+        l.aggregate = static_cast<Store*>(ws);
+        const bool native = p.out_dtype == kScanInDtype[p.acc] && p.rdt == kScanInDtype[p.acc];
+        hipLaunchKernelGGL(k.lookback[kLbTotals][native ? 1 : 0], dim3(l.total), block, 0, stream, l);
+        const bool sum = l.cpr <= kLbSumCarries;  // each chunk merges the totals before it itself
+        if (!sum) hipLaunchKernelGGL(k.lb_carry, dim3(l.rows), block, 0, stream, l);
+        hipLaunchKernelGGL(k.lb_apply[native ? 1 : 0][sum ? 1 : 0], dim3(l.total), block, 0, stream, l);
+        if (used) *used = "scan_chunks3";
+        return hipGetLastError();
+    }
     if (p.lookback) {
         LbScan l = p.lb;
         if (!ws) return hipErrorInvalidValue;
@@ -82,7 +94,7 @@ hipError_t launch_scan(const ScanPlan& p, int V, void* ws, hipStream_t stream, c
         const uint32_t init_blocks = uint32_t((n + 255) / 256);
         if (p.lb_packed) hipLaunchKernelGGL(scan_lookback_init_kernel<true>, dim3(init_blocks), block, 0, stream, l);
         else hipLaunchKernelGGL(scan_lookback_init_kernel<false>, dim3(init_blocks), block, 0, stream, l);
-        hipLaunchKernelGGL(k.lookback[native ? 1 : 0], dim3(l.total), block, 0, stream, l);
+        hipLaunchKernelGGL(k.lookback[kLbOnePass][native ? 1 : 0], dim3(l.total), block, 0, stream, l);
         if (used) *used = "scan_lookback";
         return hipGetLastError();
     }

@@ -103,6 +103,7 @@ struct ScanPlan {
     TileScan tile;
     int32_t lookback;   // 1: few long rows, the look-back kernel (then `lb` holds its shape)
     int32_t lb_packed;  // its publications are packed 64-bit words
+    int32_t lb_three;   // the same chunks in three launches (totals, carries, results) instead
     LbScan lb;
 };
 
@@ -112,6 +113,10 @@ void set_scan_blocks_per_cu(int n);
 // Measurement aid: whether the tiled scans may take the one-pass form
 // (default 1), for plans made after the call.
 void set_scan_single_pass(int on);
+// Measurement aid: whether few long rows run the one-pass look-back (1) or
+// the three-launch reduce-then-scan over the same chunks (0, the default),
+// for plans made after the call.
+void set_scan_flat_single_pass(int on);
 
 // The direct rows kernels apply: rank <= 2 rows layout with 32-bit offsets,
 // a program that is one load of a leaf of the accumulation input type, the

@@ -224,37 +224,31 @@ __device__ __forceinline__ A wave_inclusive_scan(A x, const A& id) {
 // depends on the row length only.  NATIVE: the result and the output are
 // of the input's type (the usual cumsum / cumprod), fixed at compile time
 // so that no per-element type conversion is dispatched at run time.
+// One row of `len` elements scanned from `carry` in tiles of W waves x U
+// segments (the body of the direct rows kernel; the chunked reduce-then-scan
+// runs it on each chunk from the chunk's carry).  `wr` = this wave's place in
+// a tile, `wtot` = W slots of LDS.
 template <int AK, int OP, bool NATIVE, int W>
-__global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p) {
+__device__ __forceinline__ void scan_row_tiles(const typename ScanIn<AK>::T* in, char* out, uint32_t len,
+                                               Acc<AK, OP> carry, const Acc<AK, OP>& id, bool nt_in, bool nt_out,
+                                               int odt, int rdt, int wr, Store* wtot) {
     using A = Acc<AK, OP>;
     using T = typename ScanIn<AK>::T;
     constexpr int V = int(16 / sizeof(T));
     constexpr int U = 4;
     constexpr uint32_t SEG = 64u * V;
     constexpr uint32_t TILE = SEG * U * W;
-    __shared__ Store wtot[4];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const uint32_t row = W == 1 ? blockIdx.x * 4u + uint32_t(wave) : blockIdx.x;
-    const int wr = W == 1 ? 0 : wave;  // this wave's place in a tile
-    if (row >= p.rows) return;  // uniform in a wave (W = 1) or in the block (W = 4)
-    const T* in = reinterpret_cast<const T*>(p.leaves[0].ptr) + int64_t(row) * p.leaves[0].stride[0];
-    const bool nt_in = lm_nt(p.leaves[0].mode), nt_out = lm_nt(p.out_mode);
-    const int odt = NATIVE ? int(ScanInDtype<T>::value) : p.out_dtype;
-    const int rdt = NATIVE ? int(ScanInDtype<T>::value) : p.rdt;
+    const int wave = wr;
     const int oesz = dtype_size(odt);
-    char* out = p.out_ptr + int64_t(row) * p.out_stride[0] * oesz;
-    A id;
-    id.init(p.identity);
-    A carry = id;  // everything before the current tile
-    for (uint32_t base = 0; base < p.len; base += TILE) {
+    for (uint32_t base = 0; base < len; base += TILE) {
         const uint32_t wbase = base + uint32_t(wr) * SEG * U;
         T x[U][V];
         int nv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t j = wbase + uint32_t(u) * SEG + uint32_t(lane * V);
-            nv[u] = j < p.len ? min(V, int(p.len - j)) : 0;
+            nv[u] = j < len ? min(V, int(len - j)) : 0;
             if (nv[u] == V) {
                 Reg r[V];
                 if (nt_in) load_vec<V, true>(reinterpret_cast<const char*>(in + j), int(sizeof(T)), ScanInDtype<T>::value, r);
@@ -272,7 +266,7 @@ __global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p)
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             pre[u] = local;
-            if (wbase + uint32_t(u) * SEG >= p.len) continue;  // uniform in the wave
+            if (wbase + uint32_t(u) * SEG >= len) continue;  // uniform in the wave
             A t = id;
 #pragma unroll
             for (int v = 0; v < V; ++v)
@@ -298,7 +292,7 @@ __global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p)
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (wbase + uint32_t(u) * SEG >= p.len) continue;  // uniform in the wave
+            if (wbase + uint32_t(u) * SEG >= len) continue;  // uniform in the wave
             A t = before;
             t.merge(pre[u]);
             Reg o[V];
@@ -320,6 +314,26 @@ __global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p)
             }
         }
     }
+}
+
+template <int AK, int OP, bool NATIVE, int W>
+__global__ __launch_bounds__(256) void scan_rows_direct_kernel(const ScanPlan p) {
+    using A = Acc<AK, OP>;
+    using T = typename ScanIn<AK>::T;
+    __shared__ Store wtot[4];
+    const int wave = threadIdx.x >> 6;
+    const uint32_t row = W == 1 ? blockIdx.x * 4u + uint32_t(wave) : blockIdx.x;
+    const int wr = W == 1 ? 0 : wave;  // this wave's place in a tile
+    if (row >= p.rows) return;  // uniform in a wave (W = 1) or in the block (W = 4)
+    const T* in = reinterpret_cast<const T*>(p.leaves[0].ptr) + int64_t(row) * p.leaves[0].stride[0];
+    const bool nt_in = lm_nt(p.leaves[0].mode), nt_out = lm_nt(p.out_mode);
+    const int odt = NATIVE ? int(ScanInDtype<T>::value) : p.out_dtype;
+    const int rdt = NATIVE ? int(ScanInDtype<T>::value) : p.rdt;
+    const int oesz = dtype_size(odt);
+    char* out = p.out_ptr + int64_t(row) * p.out_stride[0] * oesz;
+    A id;
+    id.init(p.identity);
+    scan_row_tiles<AK, OP, NATIVE, W>(in, out, p.len, id, id, nt_in, nt_out, odt, rdt, wr, wtot);
 }
 
 }  // namespace xg
@@ -351,7 +365,9 @@ struct ScanKernels {
     TFn tile_totals, tile_carry;
     TFn tiles[2][2];     // [native][transposed output]; the transposed one is native only
     TFn tiles1p[2][2];   // the one-pass form, likewise (32-bit accumulators only)
-    LFn lookback[2];     // [native]
+    LFn lookback[2][2];  // [kLbOnePass / kLbTotals][native]
+    LFn lb_carry;
+    LFn lb_apply[2][2];  // [native][carries summed in the kernel]
 };
 
 template <int AK, int OP> ScanKernels scan_kernels() {
@@ -368,7 +384,11 @@ template <int AK, int OP> ScanKernels scan_kernels() {
                         {scan_tiles_kernel<AK, OP, true, false>, scan_tiles_kernel<AK, OP, true, true>}},
                        {{tile_1p_fn<AK, OP, false, false>(), nullptr},
                         {tile_1p_fn<AK, OP, true, false>(), tile_1p_fn<AK, OP, true, true>()}},
-                       {scan_lookback_kernel<AK, OP, false>, scan_lookback_kernel<AK, OP, true>}};
+                       {{scan_lookback_kernel<AK, OP, false, kLbOnePass>, scan_lookback_kernel<AK, OP, true, kLbOnePass>},
+                        {scan_lookback_kernel<AK, OP, false, kLbTotals>, scan_lookback_kernel<AK, OP, true, kLbTotals>}},
+                       scan_lookback_carry_kernel<AK, OP>,
+                       {{scan_chunks_apply_kernel<AK, OP, false, false>, scan_chunks_apply_kernel<AK, OP, false, true>},
+                        {scan_chunks_apply_kernel<AK, OP, true, false>, scan_chunks_apply_kernel<AK, OP, true, true>}}};
 }
 
 bool scan_kernels_float(int ak, int op, ScanKernels* k);

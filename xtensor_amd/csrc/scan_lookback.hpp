@@ -161,7 +161,14 @@ __device__ __forceinline__ Acc<AK, OP> lb_lookback(const LbScan& p, uint32_t fir
 // One chunk of TILE = 4 waves x U segments x 64 * V elements per workgroup
 // (U = kLbSegments: 16384 float32 elements; with 4096-element chunks the
 // chain of look-backs and the ticket counter measured slower on MI355X).
-template <int AK, int OP, bool NATIVE>
+// MODE kLbOnePass: ticket, look-back, results.  The same chunks also run the
+// reduce-then-scan form without tickets or waits, in three launches:
+// kLbTotals writes each chunk's total to p.aggregate (chunk = blockIdx.x),
+// scan_lookback_carry_kernel turns them into exclusive carries in place, and
+// scan_chunks_apply_kernel scans each chunk from its carry in tiles (the
+// second read of the input is served by the last-level cache when it fits).
+constexpr int kLbOnePass = 0, kLbTotals = 1;
+template <int AK, int OP, bool NATIVE, int MODE>
 __global__ __launch_bounds__(256) void scan_lookback_kernel(const LbScan p) {
     using A = Acc<AK, OP>;
     using T = typename ScanIn<AK>::T;
@@ -173,9 +180,11 @@ __global__ __launch_bounds__(256) void scan_lookback_kernel(const LbScan p) {
     __shared__ Store s_excl;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    if (threadIdx.x == 0) s_ticket = atomicAdd(p.ctrl, 1u);
-    __syncthreads();
-    const uint32_t ticket = s_ticket;
+    if constexpr (MODE == kLbOnePass) {
+        if (threadIdx.x == 0) s_ticket = atomicAdd(p.ctrl, 1u);
+        __syncthreads();
+    }
+    const uint32_t ticket = MODE == kLbOnePass ? s_ticket : blockIdx.x;
     if (ticket >= p.total) return;  // uniform in the block (the grid holds exactly `total`)
     const uint32_t row = ticket / p.cpr;
     const uint32_t c = ticket - row * p.cpr;
@@ -197,7 +206,7 @@ __global__ __launch_bounds__(256) void scan_lookback_kernel(const LbScan p) {
         const uint32_t j = wbase + uint32_t(u) * SEG + uint32_t(lane * V);
         nv[u] = j < p.len ? min(V, int(p.len - j)) : 0;
     }
-    if (whole && nt_in) {
+    if (whole && nt_in && MODE != kLbTotals) {  // the totals pass leaves the input in the caches
 #pragma unroll
         for (int u = 0; u < U; ++u) load_row<V, true>(in + wbase + uint32_t(u) * SEG + uint32_t(lane * V), x[u]);
     } else if (whole) {
@@ -210,6 +219,28 @@ __global__ __launch_bounds__(256) void scan_lookback_kernel(const LbScan p) {
             if (nv[u] == V) load_row<V, false>(in + j, x[u]);
             else load_row_partial<V>(in + j, true, nv[u], x[u]);
         }
+    }
+    if constexpr (MODE == kLbTotals) {  // the chunk's total only: lanes in order, then the waves
+        A t = id;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+                if (v < nv[u]) t.add(reg_put<T>(x[u][v]));
+        t = lane_value<63>(wave_inclusive_scan(t, id));
+        if (lane == 0) wtot[wave] = t.store();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            A agg = id;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                A o;
+                o.load(wtot[w]);
+                agg.merge(o);
+            }
+            p.aggregate[ticket] = agg.store();
+        }
+        return;
     }
     // wave-local exclusive prefixes of the segments, and the wave's total
     A local = id;
@@ -280,6 +311,100 @@ __global__ __launch_bounds__(256) void scan_lookback_kernel(const LbScan p) {
                 if (v < nv[u]) store_raw(ptr + int64_t(v) * oesz, oesz, o[v]);
         }
     }
+}
+
+// In place: the chunk totals of every row (p.aggregate[row * cpr + c])
+// become exclusive carries.  A workgroup per row walks its totals in tiles of
+// 256, a thread each: a wave scan, the wave totals through LDS, and the tile
+// total carried into the next tile.
+template <int AK, int OP>
+__global__ __launch_bounds__(256) void scan_lookback_carry_kernel(const LbScan p) {
+    using A = Acc<AK, OP>;
+    __shared__ Store wtot[4];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    Store* t = p.aggregate + uint64_t(blockIdx.x) * p.cpr;
+    A id;
+    id.init(p.identity);
+    A carry = id;
+    for (uint32_t base = 0; base < p.cpr; base += 256u) {  // uniform in the block
+        const uint32_t j = base + threadIdx.x;
+        A x = id;
+        if (j < p.cpr) x.load(t[j]);
+        const A incl = wave_inclusive_scan(x, id);
+        A excl = dpp_move<0x138, 0xf>(incl, id);  // wave_shr:1, lane 0 takes the identity
+        if (lane == 63) wtot[wave] = incl.store();
+        __syncthreads();
+        A before = carry;
+        A next = carry;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            A o;
+            o.load(wtot[w]);
+            if (w == wave) before = next;
+            next.merge(o);
+        }
+        before.merge(excl);
+        if (j < p.cpr) t[j] = before.store();
+        carry = next;
+        __syncthreads();  // wtot is written again for the next tile
+    }
+}
+
+// Each chunk (a workgroup) scanned from its exclusive carry, streaming in
+// tiles of 4 waves x 4 segments as the direct rows kernel does (only a tile's
+// values in registers at a time).  SUM: the carry is the merge of the chunk
+// totals before it in its row (p.aggregate, read from the L2: at most
+// kLbSumCarries of them; thread t takes totals t, t + 256, ..., then a wave
+// scan and the four waves in order); else p.aggregate already holds the
+// exclusive carries (scan_lookback_carry_kernel).
+constexpr uint32_t kLbSumCarries = 4096;
+template <int AK, int OP, bool NATIVE, bool SUM>
+__global__ __launch_bounds__(256) void scan_chunks_apply_kernel(const LbScan p) {
+    using A = Acc<AK, OP>;
+    using T = typename ScanIn<AK>::T;
+    __shared__ Store wtot[4];
+    const uint32_t chunk = blockIdx.x;
+    if (chunk >= p.total) return;  // uniform in the block
+    const uint32_t row = chunk / p.cpr;
+    const uint32_t c = chunk - row * p.cpr;
+    const int odt = NATIVE ? int(ScanInDtype<T>::value) : p.out_dtype;
+    const int rdt = NATIVE ? int(ScanInDtype<T>::value) : p.rdt;
+    const int oesz = dtype_size(odt);
+    const int64_t off = int64_t(c) * p.tile;
+    const T* in = reinterpret_cast<const T*>(p.in) + int64_t(row) * p.in_row_stride + off;
+    char* out = p.out + (int64_t(row) * p.out_row_stride + off) * oesz;
+    const uint32_t len = min(p.tile, p.len - c * p.tile);
+    A id;
+    id.init(p.identity);
+    A carry;
+    if constexpr (SUM) {
+        const int lane = threadIdx.x & 63;
+        const int wave = threadIdx.x >> 6;
+        const Store* tot = p.aggregate + uint64_t(row) * p.cpr;
+        A acc = id;
+#pragma unroll 4
+        for (uint32_t j = threadIdx.x; j < c; j += 256u) {
+            A o;
+            o.load(tot[j]);
+            acc.merge(o);
+        }
+        acc = lane_value<63>(wave_inclusive_scan(acc, id));
+        if (lane == 0) wtot[wave] = acc.store();
+        __syncthreads();
+        carry = id;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            A o;
+            o.load(wtot[w]);
+            carry.merge(o);
+        }
+        __syncthreads();  // wtot is used again by the tiles
+    } else {
+        carry.load(p.aggregate[chunk]);
+    }
+    scan_row_tiles<AK, OP, NATIVE, 4>(in, out, len, carry, id, false, p.nt_out != 0, odt, rdt,
+                                      int(threadIdx.x >> 6), wtot);
 }
 
 // clears the ticket counter, the timeout count and every chunk's flag (a

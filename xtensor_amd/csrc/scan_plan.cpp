@@ -67,6 +67,7 @@ namespace {
 
 int g_tile_blocks_per_cu = 8;  // workgroups per compute unit the tiled scans aim at
 int g_tile_single_pass = 1;    // the one-pass tiled form may apply
+int g_flat_single_pass = 0;    // few long rows: one-pass look-back (1) or three launches (0)
 
 // element type the direct and tiled kernels read, per accumulator kind
 int scan_in_dtype(int acc) {
@@ -192,6 +193,7 @@ void plan_lookback(ScanPlan* pp, int cus) {
     l.nt_in = lm_nt(p.leaves[0].mode) ? 1 : 0;
     l.nt_out = lm_nt(p.out_mode) ? 1 : 0;
     p.lb_packed = (p.acc == AK_F32C || p.acc == AK_F32 || p.acc == AK_I32 || p.acc == AK_U32) ? 1 : 0;
+    p.lb_three = g_flat_single_pass ? 0 : 1;
     p.lookback = 1;
 }
 
@@ -201,6 +203,7 @@ size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 
 void set_scan_blocks_per_cu(int n) { g_tile_blocks_per_cu = n > 0 ? n : 8; }
 void set_scan_single_pass(int on) { g_tile_single_pass = on ? 1 : 0; }
+void set_scan_flat_single_pass(int on) { g_flat_single_pass = on ? 1 : 0; }
 
 bool scan_direct_ok(const ScanPlan& p) {
     // rows are addressed with 64-bit offsets and elements within a row with
@@ -221,6 +224,7 @@ size_t scan_workspace_bytes(const ScanPlan& p) {
     if (p.rows == 0) return 0;
     if (p.lookback) {
         const size_t n = p.lb.total;
+        if (p.lb_three) return align256(n * sizeof(Store));
         if (p.lb_packed) return 256 + align256(n * 8);
         return 256 + align256(n * 4) + 2 * align256(n * sizeof(Store));
     }
