@@ -124,7 +124,7 @@ void plan_tiled(ScanPlan* pp, int cus) {
         return;
     }
     if (t.in_s_outer % V || t.in_s_len % V || t.in_s_len < 0 || t.in_s_outer < 0) return;
-    const uint32_t sub = kind == ST_TRANS ? 32u : 64u;  // rows of one sub-tile (scan_tiles_kernel)
+    const uint32_t sub = 64u;  // rows of one sub-tile (scan_tiles_kernel)
     const uint64_t cols = uint64_t(64 * V);
     t.ntiles = uint32_t((t.inner + cols - 1) / cols);
     t.ncols = uint32_t(t.ntiles * cols);

@@ -163,13 +163,14 @@ __global__ __launch_bounds__(256) void scan_tiles_kernel(const TileScan p) {
     using T = typename ScanIn<AK>::T;
     constexpr int V = int(16 / sizeof(T));
     constexpr int COLS = 64 * V;
-    constexpr int RW = TRANS ? 8 : 16;  // rows per wave in a sub-tile
+    constexpr int RW = 16;  // rows per wave in a sub-tile
     constexpr uint32_t SUB = 4u * RW;
+    constexpr int HC = COLS / 2;         // TRANS: columns of one half of the tile in LDS
     constexpr int PITCH = int(SUB) + 1;  // output tile row pitch (TRANS), in elements
     static_assert(!TRANS || NATIVE, "the transposed scan stores the input's type");
     union Shared {
         Store wtot[4][COLS];
-        T tile[TRANS ? COLS : 1][TRANS ? PITCH : 1];
+        T tile[TRANS ? HC : 1][TRANS ? PITCH : 1];
     };
     __shared__ Shared sm;
     const int lane = threadIdx.x & 63;
@@ -268,38 +269,51 @@ __global__ __launch_bounds__(256) void scan_tiles_kernel(const TileScan p) {
                 }
             }
         } else {
-            // the tile as [column][row] in LDS, then stores along the output's rows
+            // the results in place of the values, then the sub-tile as [column][row]
+            // in LDS in two halves of the columns (lanes 0-31, then 32-63), each
+            // stored along the output's rows: a column's 64 rows leave as one
+            // segment of 64 elements (32-row sub-tiles, half that, measured slower
+            // on the one-pass form)
 #pragma unroll
             for (int i = 0; i < RW; ++i) {
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
                     if (rw0 + uint32_t(i) < r1) run[v].add(reg_put<T>(x[i][v]));
-                    sm.tile[lane * V + v][wave * RW + i] = reg_get<T>(run[v].value(rdt));
+                    x[i][v] = reg_get<T>(run[v].value(rdt));
                 }
             }
-            __syncthreads();
             constexpr int TPC = int(SUB) / V;  // threads per column: 16 bytes each
             constexpr int CPP = 256 / TPC;     // columns per pass
             const int cc = threadIdx.x / TPC;
             const int rr = (threadIdx.x - cc * TPC) * V;
             const uint32_t row = sub + uint32_t(rr);
 #pragma unroll
-            for (int c = cc; c < COLS; c += CPP) {
-                const uint32_t gcol = tile * uint32_t(COLS) + uint32_t(c);
-                if (gcol >= p.inner || row >= r1) continue;
-                char* ptr = obase + (int64_t(row) * p.out_s_len + int64_t(gcol) * p.out_s_inner) * oesz;
-                Reg q[V];
+            for (int h = 0; h < 2; ++h) {
+                if ((lane >> 5) == h) {
 #pragma unroll
-                for (int v = 0; v < V; ++v) q[v] = reg_put<T>(sm.tile[c][rr + v]);
-                if (row + uint32_t(V) <= r1) {
-                    store_vec_rt(nt_out, ptr, oesz, odt, q);
-                } else {
+                    for (int i = 0; i < RW; ++i)
 #pragma unroll
-                    for (int v = 0; v < V; ++v)
-                        if (row + uint32_t(v) < r1) store_raw(ptr + int64_t(v) * oesz, oesz, q[v]);
+                        for (int v = 0; v < V; ++v) sm.tile[(lane & 31) * V + v][wave * RW + i] = x[i][v];
                 }
+                __syncthreads();
+#pragma unroll
+                for (int c = cc; c < HC; c += CPP) {
+                    const uint32_t gcol = tile * uint32_t(COLS) + uint32_t(h * HC + c);
+                    if (gcol >= p.inner || row >= r1) continue;
+                    char* ptr = obase + (int64_t(row) * p.out_s_len + int64_t(gcol) * p.out_s_inner) * oesz;
+                    Reg q[V];
+#pragma unroll
+                    for (int v = 0; v < V; ++v) q[v] = reg_put<T>(sm.tile[c][rr + v]);
+                    if (row + uint32_t(V) <= r1) {
+                        store_vec_rt(nt_out, ptr, oesz, odt, q);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < V; ++v)
+                            if (row + uint32_t(v) < r1) store_raw(ptr + int64_t(v) * oesz, oesz, q[v]);
+                    }
+                }
+                __syncthreads();  // the tile space is written again (second half, next sub-tile)
             }
-            __syncthreads();  // the tile space is written again next sub-tile
         }
     }
 }
