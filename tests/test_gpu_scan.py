@@ -287,13 +287,15 @@ def test_tiled_scan_store_conversion_and_plans():
         assert np.array_equal(o2.to_numpy(), np.cumsum(y.astype(np.float64), axis=0).astype(np.float32))
 
 
-# (shape, axis, transposed input): enough column tiles (outer x tiles >= 8;
-# 9, 10, 12 and 16 of them: one, two, four and eight ticket counters)
-# for the one-pass form; partial column tiles, partial last chunks, a 3-d
-# outer dim and chains of up to 313 chunks (many look-back windows)
+# (shape, axis, transposed input) for the one-pass form: 1 to 16 column
+# chains (outer x tiles; 1, 2, 4, 8 and 16 ticket counters), partial column
+# tiles, partial last chunks, a 3-d outer dim and chains of up to 2048 chunks
+# (many look-back windows)
 ONE_PASS_CASES = [((8192, 4096), 0, False), ((300, 2100), 0, False), ((4, 200, 520), 1, False),
                   ((20000, 2048), 0, False), ((65, 2304), 0, False), ((130, 2560), 0, False),
-                  ((8192, 4096), 1, True), ((600, 2100), 1, True), ((100, 2052), 1, True)]
+                  ((131072, 200), 0, False), ((3, 5000, 256), 1, False), ((1000, 512), 0, False),
+                  ((8192, 4096), 1, True), ((600, 2100), 1, True), ((100, 2052), 1, True),
+                  ((30000, 100), 1, True)]
 
 
 def _one_pass_run(x, axis, trans, kind):
@@ -307,8 +309,8 @@ def _one_pass_run(x, axis, trans, kind):
 @pytest.mark.parametrize("dtype", ["float32", "int32", "uint32"])
 @pytest.mark.parametrize("kind", ["cumsum", "cumprod"])
 def test_tiled_scans_one_pass(dtype, kind):
-    """Strided-axis scans of 32-bit accumulators over at least eight column
-    tiles run in one pass: every chunk of 64 rows is read once, its column
+    """Strided-axis scans of 32-bit accumulators over two or more chunks of
+    the axis run in one pass: every chunk of 64 rows is read once, its column
     totals are published and each column looks back across the preceding
     chunks for its carry.  Checked against the sequential reference (integers
     exact, float32 sums rel 2e-7, products n ulps) and against the

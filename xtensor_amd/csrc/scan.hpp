@@ -44,10 +44,8 @@ struct TileScan {
     uint64_t* words;      // [outer][nchunks][ncols] packed value + flag
 };
 
-// Rows of one chunk of the one-pass tiled scan (held in registers whole),
-// and the fewest column tiles (outer x tiles) it takes.
+// Rows of one chunk of the one-pass tiled scan (held in registers whole).
 constexpr uint32_t kTileLbRows = 64;
-constexpr uint64_t kTileLbMinTiles = 8;
 // its ticket counters (at most kTileMaxClasses, 256 bytes apart) and timeout
 // count at the head of the workspace
 constexpr uint32_t kTileMaxClasses = 16;

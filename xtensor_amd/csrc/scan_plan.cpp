@@ -139,13 +139,13 @@ void plan_tiled(ScanPlan* pp, int cus) {
     t.nchunks = uint32_t((uint64_t(t.len) + t.chunk_rows - 1) / t.chunk_rows);
     if (per_chunk * t.nchunks >= (uint64_t(1) << 31)) return;
     // One pass with a look-back per column: 32-bit accumulators (published
-    // packed with their flag), at least two chunks, and enough column tiles
-    // side by side (the front of published prefixes advances a window of
-    // chunks per look-back round trip in each tile; measured on MI355X).
+    // packed with their flag) and at least two chunks.  Measured on MI355X
+    // faster than the three launches from one column tile up (float32 cumsum
+    // along axis 0: [131072, 256] 99 against 124 us, [32768, 1024] 71 against
+    // 88 us, [8192, 4096] 67 against 81 us).
     const bool packed = p.acc == AK_F32C || p.acc == AK_F32 || p.acc == AK_I32 || p.acc == AK_U32;
     const uint64_t chunks1 = (uint64_t(t.len) + kTileLbRows - 1) / kTileLbRows;
-    if (g_tile_single_pass && packed && per_chunk >= kTileLbMinTiles && chunks1 >= 2 &&
-        per_chunk * chunks1 < (uint64_t(1) << 31)) {
+    if (g_tile_single_pass && packed && chunks1 >= 2 && per_chunk * chunks1 < (uint64_t(1) << 31)) {
         t.single = 1;
         t.chunk_rows = kTileLbRows;
         t.nchunks = uint32_t(chunks1);

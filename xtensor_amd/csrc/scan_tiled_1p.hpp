@@ -32,9 +32,6 @@
 // never a hang).  The look-back merges in an order that depends on timing, so
 // float sums can differ in the last bit between runs (integers are exact).
 //
-// The chains of the column tiles advance side by side, so the planner takes
-// this form only when there are enough tiles.
-//
 // Included by scan_kernels.hpp, after scan_tiled.hpp and scan_lookback.hpp.
 #pragma once
 
