@@ -306,16 +306,17 @@ def _one_pass_run(x, axis, trans, kind):
     return got.to_numpy(), xt.last_kernel()
 
 
-@pytest.mark.parametrize("dtype", ["float32", "int32", "uint32"])
+@pytest.mark.parametrize("dtype", ["float32", "int32", "uint32", "float64", "int64"])
 @pytest.mark.parametrize("kind", ["cumsum", "cumprod"])
 def test_tiled_scans_one_pass(dtype, kind):
-    """Strided-axis scans of 32-bit accumulators over two or more chunks of
-    the axis run in one pass: every chunk of 64 rows is read once, its column
-    totals are published and each column looks back across the preceding
-    chunks for its carry.  Checked against the sequential reference (integers
-    exact, float32 sums rel 2e-7, products n ulps) and against the
-    three-launch form of the same scan (the knob that turns the one pass
-    off)."""
+    """Strided-axis scans over two or more chunks of the axis run in one
+    pass: every chunk of 64 rows is read once, its column totals are
+    published and each column looks back across the preceding chunks for its
+    carry (32-bit accumulators: value and flag in one word; 64-bit ones:
+    value, wait, then one flag per chunk and tile).  Checked against the
+    sequential reference (integers exact, float sums rel 2e-7 / 1e-14,
+    products n ulps) and against the three-launch form of the same scan (the
+    knob that turns the one pass off)."""
     from xtensor_amd import _abi
     lib = _abi.lib()
     rng = np.random.default_rng(35)
@@ -336,7 +337,8 @@ def test_tiled_scans_one_pass(dtype, kind):
         assert name3 == want[:-3], (shape, name3)
         if np.dtype(dtype).kind == "f":
             scale = np.maximum(np.abs(three.astype(np.float64)), 1.0)
-            bar = 2e-7 if kind == "cumsum" else 2 * src.shape[axis] * 2.0 ** -24
+            ulp = 2.0 ** -24 if dtype == "float32" else 2.0 ** -53
+            bar = (2e-7 if dtype == "float32" else 1e-14) if kind == "cumsum" else 2 * src.shape[axis] * ulp
             assert np.max(np.abs(got.astype(np.float64) - three) / scale) <= bar, (kind, shape)
         else:
             assert np.array_equal(got, three), (kind, shape)

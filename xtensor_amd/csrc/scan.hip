@@ -43,11 +43,26 @@ hipError_t launch_scan(const ScanPlan& p, int V, void* ws, hipStream_t stream, c
             const uint64_t total = uint64_t(t.outer) * t.nchunks * t.ntiles;
             if (t.classes < 1 || t.classes > kTileMaxClasses || (uint64_t(t.outer) * t.ntiles) % t.classes)
                 return hipErrorInvalidValue;
-            t.ctrl = static_cast<uint32_t*>(ws);
-            t.words = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + kTileCtrlBytes);
-            const uint64_t n2 = uint64_t(t.outer) * t.nchunks * t.ncols / 2;  // ncols: a multiple of 256
+            auto a256 = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
+            char* w = static_cast<char*>(ws);
+            t.ctrl = reinterpret_cast<uint32_t*>(w);
+            w += kTileCtrlBytes;
+            const uint64_t n = uint64_t(t.outer) * t.nchunks * t.ncols;
+            uint64_t n2;  // 16-byte words the init kernel clears: the packed words, or the flags
+            if (t.packed) {
+                t.words = reinterpret_cast<uint64_t*>(w);
+                n2 = n / 2;  // ncols: a multiple of 128
+            } else {
+                const uint64_t nf = uint64_t(t.outer) * t.nchunks * t.ntiles;
+                t.flags = reinterpret_cast<uint32_t*>(w);
+                w += a256(nf * 4);
+                t.agg = reinterpret_cast<Store*>(w);
+                w += a256(n * sizeof(Store));
+                t.pre = reinterpret_cast<Store*>(w);
+                n2 = (nf + 3) / 4;  // within the flags' 256-byte aligned block
+            }
             hipLaunchKernelGGL(scan_tiles_1p_init_kernel, dim3(uint32_t((std::max<uint64_t>(n2, 16) + 255) / 256)), block, 0, stream, t.ctrl,
-                               reinterpret_cast<uint4*>(t.words), n2);
+                               reinterpret_cast<uint4*>(t.packed ? static_cast<void*>(t.words) : static_cast<void*>(t.flags)), n2);
             hipLaunchKernelGGL(one, dim3(uint32_t(total)), block, 0, stream, t);
             if (used) *used = trans ? "scan_tiles_trans_1p" : "scan_tiles_1p";
             return hipGetLastError();

@@ -41,7 +41,11 @@ struct TileScan {
     int32_t single;
     uint32_t classes;     // ticket counters: chain k (= o * ntiles + tile) draws from counter k % classes
     uint32_t* ctrl;       // counter k at ctrl[k * kTileCtrlStride]; timeouts at ctrl[kTileTimeouts]
-    uint64_t* words;      // [outer][nchunks][ncols] packed value + flag
+    uint64_t* words;      // 32-bit accumulators: [outer][nchunks][ncols] packed value + flag
+    uint32_t* flags;      // 64-bit accumulators: [outer][nchunks][ntiles] flag ...
+    Store* agg;           // ... [outer][nchunks][ncols] column totals
+    Store* pre;           // ... and inclusive prefixes
+    int32_t packed;       // 1: `words`; 0: `flags`, `agg`, `pre`
 };
 
 // Rows of one chunk of the one-pass tiled scan (held in registers whole).

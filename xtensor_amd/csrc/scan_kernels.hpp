@@ -352,9 +352,9 @@ using SFn = void (*)(const ScanPlan);
 using TFn = void (*)(const TileScan);
 using LFn = void (*)(const LbScan);
 
-// the one-pass tiled kernel where the accumulator publishes packed (else null)
+// the one-pass tiled kernel (the transposed form stores the input's type only)
 template <int AK, int OP, bool NATIVE, bool TRANS> constexpr TFn tile_1p_fn() {
-    if constexpr (LbPacked<AK>::value && (NATIVE || !TRANS)) return scan_tiles_1p_kernel<AK, OP, NATIVE, TRANS>;
+    if constexpr (NATIVE || !TRANS) return scan_tiles_1p_kernel<AK, OP, NATIVE, TRANS>;
     else return nullptr;
 }
 
@@ -364,7 +364,7 @@ struct ScanKernels {
     uint32_t wave_tile;  // elements of one wave's tile in the direct kernels
     TFn tile_totals, tile_carry;
     TFn tiles[2][2];     // [native][transposed output]; the transposed one is native only
-    TFn tiles1p[2][2];   // the one-pass form, likewise (32-bit accumulators only)
+    TFn tiles1p[2][2];   // the one-pass form, likewise
     LFn lookback[2][2];  // [kLbOnePass / kLbTotals][native]
     LFn lb_carry;
     LFn lb_apply[2][2];  // [native][carries summed in the kernel]

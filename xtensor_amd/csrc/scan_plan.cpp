@@ -138,14 +138,16 @@ void plan_tiled(ScanPlan* pp, int cus) {
     t.chunk_rows = uint32_t((rows + sub - 1) / sub * sub);
     t.nchunks = uint32_t((uint64_t(t.len) + t.chunk_rows - 1) / t.chunk_rows);
     if (per_chunk * t.nchunks >= (uint64_t(1) << 31)) return;
-    // One pass with a look-back per column: 32-bit accumulators (published
-    // packed with their flag) and at least two chunks.  Measured on MI355X
-    // faster than the three launches from one column tile up (float32 cumsum
-    // along axis 0: [131072, 256] 99 against 124 us, [32768, 1024] 71 against
-    // 88 us, [8192, 4096] 67 against 81 us).
+    // One pass with a look-back per column when there are at least two
+    // chunks.  Measured on MI355X faster than the three launches from one
+    // column tile up (float32 cumsum along axis 0: [131072, 256] 99 against
+    // 124 us, [32768, 1024] 71 against 88 us, [8192, 4096] 67 against 81 us).
+    // 32-bit accumulators publish packed with their flag, 64-bit ones as
+    // value, wait, flag per (chunk, tile).
     const bool packed = p.acc == AK_F32C || p.acc == AK_F32 || p.acc == AK_I32 || p.acc == AK_U32;
     const uint64_t chunks1 = (uint64_t(t.len) + kTileLbRows - 1) / kTileLbRows;
-    if (g_tile_single_pass && packed && chunks1 >= 2 && per_chunk * chunks1 < (uint64_t(1) << 31)) {
+    if (g_tile_single_pass && chunks1 >= 2 && per_chunk * chunks1 < (uint64_t(1) << 31)) {
+        t.packed = packed ? 1 : 0;
         t.single = 1;
         t.chunk_rows = kTileLbRows;
         t.nchunks = uint32_t(chunks1);
@@ -229,7 +231,12 @@ size_t scan_workspace_bytes(const ScanPlan& p) {
         return 256 + align256(n * 4) + 2 * align256(n * sizeof(Store));
     }
     if (p.tiled == ST_NONE || p.tile.nchunks <= 1) return 0;
-    if (p.tile.single) return kTileCtrlBytes + align256(size_t(p.tile.outer) * p.tile.nchunks * p.tile.ncols * 8);
+    if (p.tile.single) {
+        const size_t n = size_t(p.tile.outer) * p.tile.nchunks * p.tile.ncols;
+        if (p.tile.packed) return kTileCtrlBytes + align256(n * 8);
+        const size_t nf = size_t(p.tile.outer) * p.tile.nchunks * p.tile.ntiles;
+        return kTileCtrlBytes + align256(nf * 4) + 2 * align256(n * sizeof(Store));
+    }
     return size_t(p.tile.outer) * p.tile.nchunks * p.tile.ncols * sizeof(Store);
 }
 

@@ -123,13 +123,90 @@ __device__ __forceinline__ Acc<AK, OP> tile_follow(uint64_t* col, uint32_t strid
     return excl;
 }
 
+// 64-bit accumulators do not fit beside a flag in one word: each (chunk,
+// tile) has one flag word, and the columns' totals and inclusive prefixes go
+// to two arrays of their own.  The workgroup stores its columns' values with
+// device-scope atomic stores, waits for them to complete, synchronises, and
+// only then one thread sets the flag (value, wait, flag); a value array is
+// written once per launch, so a reader that saw a flag reads complete values.
+// The leader polls the flags, 64 preceding chunks at once, for the depth d of
+// the nearest published prefix (as tile_lead_depth does on column 0's words);
+// `*timed_out` tells the followers that slot d never published (it then
+// counts as a prefix of the identity).
+__device__ __forceinline__ uint32_t tile_lead_depth_flags(uint32_t* flags, uint32_t stride, uint32_t c,
+                                                          bool* timed_out) {
+    const int lane = threadIdx.x & 63;
+    int64_t nearest = int64_t(c) - 1;
+    uint32_t depth = 0;
+    *timed_out = false;
+    while (true) {
+        const int64_t j = nearest - lane;
+        uint32_t f = j >= 0 ? 0u : kLbPrefix;  // before chunk 0: the identity, as a prefix
+        int spins = 0;
+        int lim = 64;
+        while (true) {
+            if (f == 0) f = lb_load(flags + j * int64_t(stride));
+            const uint64_t pm = __ballot(f == kLbPrefix);
+            lim = pm ? __builtin_ctzll(pm) : 64;
+            const uint64_t zm = __ballot(f == 0 && lane < lim);
+            if (zm == 0) break;
+            if (++spins >= kSpinLimit) {
+                lim = __builtin_ctzll(zm);
+                *timed_out = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (lim < 64) return depth + uint32_t(lim);
+        depth += 64;
+        nearest -= 64;
+    }
+}
+
+// A follower of the flagged form: the merge of the aggregates of chunks
+// c-1 ... c-d and the inclusive prefix of chunk c-1-d (the identity where
+// that lies before chunk 0 or never published), all from complete values.
+template <int AK, int OP>
+__device__ __forceinline__ Acc<AK, OP> tile_follow_values(const Store* agg, const Store* pre, uint32_t stride,
+                                                          uint32_t c, uint32_t d, bool timed_out,
+                                                          const Acc<AK, OP>& id) {
+    using A = Acc<AK, OP>;
+    A excl = id;
+    for (uint32_t m = 0; m <= d; ++m) {
+        const int64_t j = int64_t(c) - 1 - int64_t(m);
+        if (j < 0 || (m == d && timed_out)) break;  // the identity
+        const Store* src = (m == d ? pre : agg) + j * int64_t(stride);
+        Store st{lb_load(&src->a), lb_load(&src->b)};
+        A v;
+        v.load(st);
+        v.merge(excl);  // this chunk lies before what was merged so far
+        excl = v;
+    }
+    return excl;
+}
+
+// the column's values, then (after every thread's stores completed) the flag
+template <int AK, int OP>
+__device__ __forceinline__ void tile_publish_flagged(Store* dst, bool has_col, const Acc<AK, OP>& a, uint32_t* flag,
+                                                     uint32_t f) {
+    if (has_col) {
+        const Store st = a.store();
+        lb_store(&dst->a, st.a);
+        lb_store(&dst->b, st.b);
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // this thread's stores have completed
+    __syncthreads();
+    if (threadIdx.x == 0) lb_store(flag, f);
+}
+
 template <int AK, int OP, bool NATIVE, bool TRANS>
 __global__ __launch_bounds__(256) void scan_tiles_1p_kernel(const TileScan p) {
     using A = Acc<AK, OP>;
     using T = typename ScanIn<AK>::T;
     constexpr int V = int(16 / sizeof(T));
     constexpr int COLS = 64 * V;
-    static_assert(COLS == 256, "one column of the tile per thread in the look-back");
+    constexpr bool PACKED = LbPacked<AK>::value;
+    static_assert(COLS == 256 || !PACKED, "one column of the tile per thread in the packed look-back");
     constexpr int CH = int(kTileLbRows);
     constexpr int RW = CH / 4;                  // rows per wave in a sub-tile
     constexpr uint32_t SUB = 4u * RW;           // rows of a sub-tile
@@ -144,7 +221,7 @@ __global__ __launch_bounds__(256) void scan_tiles_1p_kernel(const TileScan p) {
     };
     __shared__ Shared sm;
     __shared__ Store s_excl[COLS];
-    __shared__ uint32_t s_ticket, s_depth;
+    __shared__ uint32_t s_ticket, s_depth, s_timed_out;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const uint32_t cls = blockIdx.x % p.classes;
@@ -207,7 +284,48 @@ __global__ __launch_bounds__(256) void scan_tiles_1p_kernel(const TileScan p) {
         }
     }
     __syncthreads();
-    {  // thread c: column c of the tile, the chunk's total, look-back, publications
+    if constexpr (!PACKED) {  // thread c < COLS: column c; value, wait, flag per (chunk, tile)
+        const int c = int(threadIdx.x);
+        const bool has_col = c < COLS;
+        A agg = id;
+        if (has_col) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    A t;
+                    t.load(sm.wtot[s][w][c]);
+                    agg.merge(t);
+                }
+        }
+        const int64_t vbase = int64_t(o) * p.nchunks * p.ncols + tile * uint32_t(COLS) + (has_col ? c : 0);
+        uint32_t* ftile = p.flags + int64_t(o) * p.nchunks * p.ntiles + tile;
+        uint32_t* fmine = ftile + int64_t(chunk) * p.ntiles;
+        const int64_t vmine = vbase + int64_t(chunk) * p.ncols;
+        A excl = id;
+        if (chunk == 0) {  // uniform in the block
+            tile_publish_flagged<AK, OP>(p.pre + vmine, has_col, agg, fmine, kLbPrefix);
+        } else {
+            tile_publish_flagged<AK, OP>(p.agg + vmine, has_col, agg, fmine, kLbAggregate);
+            if (wave == 0) {
+                bool to = false;
+                const uint32_t d = tile_lead_depth_flags(ftile, p.ntiles, chunk, &to);
+                if (lane == 0) {
+                    s_depth = d;
+                    s_timed_out = to ? 1u : 0u;
+                    if (to) atomicAdd(p.ctrl + kTileTimeouts, 1u);
+                }
+            }
+            __syncthreads();
+            if (has_col)
+                excl = tile_follow_values<AK, OP>(p.agg + vbase, p.pre + vbase, p.ncols, chunk, s_depth,
+                                                  s_timed_out != 0, id);
+            A incl = excl;
+            incl.merge(agg);
+            tile_publish_flagged<AK, OP>(p.pre + vmine, has_col, incl, fmine, kLbPrefix);
+        }
+        if (has_col) s_excl[c] = excl.store();
+    } else {  // thread c: column c of the tile, the chunk's total, look-back, publications
         const int c = int(threadIdx.x);
         A agg = id;
 #pragma unroll
