@@ -199,34 +199,30 @@ __global__ __launch_bounds__(256) void scan_lookback_kernel(const LbScan p) {
     const uint32_t wbase = c * p.tile + uint32_t(wave) * SEG * U;
     // a whole chunk (all but a row's last) loads without a branch between the loads
     const bool whole = (c + 1u) * p.tile <= p.len;  // uniform in the block
-    T x[U][V];
-    int nv[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const uint32_t j = wbase + uint32_t(u) * SEG + uint32_t(lane * V);
-        nv[u] = j < p.len ? min(V, int(p.len - j)) : 0;
-    }
-    if (whole && nt_in && MODE != kLbTotals) {  // the totals pass leaves the input in the caches
-#pragma unroll
-        for (int u = 0; u < U; ++u) load_row<V, true>(in + wbase + uint32_t(u) * SEG + uint32_t(lane * V), x[u]);
-    } else if (whole) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) load_row<V, false>(in + wbase + uint32_t(u) * SEG + uint32_t(lane * V), x[u]);
-    } else {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t j = wbase + uint32_t(u) * SEG + uint32_t(lane * V);
-            if (nv[u] == V) load_row<V, false>(in + j, x[u]);
-            else load_row_partial<V>(in + j, true, nv[u], x[u]);
-        }
-    }
-    if constexpr (MODE == kLbTotals) {  // the chunk's total only: lanes in order, then the waves
+    if constexpr (MODE == kLbTotals) {
+        // the chunk's total only, streaming G segments a step (fewer registers
+        // than holding the chunk: more waves in flight); lanes in order, then
+        // the waves; default cache policy, so the second read can come from
+        // the last-level cache
+        constexpr int G = 8;
         A t = id;
+#pragma unroll 1
+        for (int g = 0; g < U; g += G) {
+            T xs[G][V];
+            int nvs[G];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
+            for (int k = 0; k < G; ++k) {
+                const uint32_t j = wbase + uint32_t(g + k) * SEG + uint32_t(lane * V);
+                nvs[k] = j < p.len ? min(V, int(p.len - j)) : 0;
+                if (whole || nvs[k] == V) load_row<V, false>(in + j, xs[k]);
+                else load_row_partial<V>(in + j, true, nvs[k], xs[k]);
+            }
 #pragma unroll
-            for (int v = 0; v < V; ++v)
-                if (v < nv[u]) t.add(reg_put<T>(x[u][v]));
+            for (int k = 0; k < G; ++k)
+#pragma unroll
+                for (int v = 0; v < V; ++v)
+                    if (v < nvs[k]) t.add(reg_put<T>(xs[k][v]));
+        }
         t = lane_value<63>(wave_inclusive_scan(t, id));
         if (lane == 0) wtot[wave] = t.store();
         __syncthreads();
@@ -241,6 +237,27 @@ __global__ __launch_bounds__(256) void scan_lookback_kernel(const LbScan p) {
             p.aggregate[ticket] = agg.store();
         }
         return;
+    }
+    T x[U][V];
+    int nv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t j = wbase + uint32_t(u) * SEG + uint32_t(lane * V);
+        nv[u] = j < p.len ? min(V, int(p.len - j)) : 0;
+    }
+    if (whole && nt_in) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) load_row<V, true>(in + wbase + uint32_t(u) * SEG + uint32_t(lane * V), x[u]);
+    } else if (whole) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) load_row<V, false>(in + wbase + uint32_t(u) * SEG + uint32_t(lane * V), x[u]);
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t j = wbase + uint32_t(u) * SEG + uint32_t(lane * V);
+            if (nv[u] == V) load_row<V, false>(in + j, x[u]);
+            else load_row_partial<V>(in + j, true, nv[u], x[u]);
+        }
     }
     // wave-local exclusive prefixes of the segments, and the wave's total
     A local = id;
