@@ -4,8 +4,8 @@
 // accumulator_impl (reducers/xaccumulator.hpp:240-297) runs the recurrence
 // res[k] = res[k-1] + res[k] down the axis for every column.  The three
 // launches of scan_tiled.hpp read the input twice; here a workgroup holds its
-// whole chunk (64 rows of one 256-column tile, 16 values per lane) in
-// registers, so the input is read once:
+// whole chunk (64 rows of one tile of 64 lanes x 16 bytes: 256 bytes a lane)
+// in registers, so the input is read once:
 //   * a ticket from an atomic counter orders the workgroups (chunk-major), so
 //     the chunk before this one in the same column tile was taken earlier and
 //     is resident or finished when this one waits for it.  The column tiles
@@ -23,11 +23,12 @@
 //     inclusive prefix;
 //   * the chunk is scanned from those prefixes and stored, as in the scan
 //     phase of the three-launch form.
-// Publications are one 64-bit word per (chunk, column) holding the value and
-// its flag (the packed form of scan_lookback.hpp, so only 32-bit
-// accumulators: float32 sums with compensation, float32 products, 32-bit
-// integers), written and read with single device-scope atomic accesses: no
-// fence.  Every wait is bounded (kSpinLimit polls; then the chunk counts as a
+// Publications of 32-bit accumulators (float32 sums with compensation,
+// float32 products, 32-bit integers) are one 64-bit word per (chunk, column)
+// holding the value and its flag (the packed form of scan_lookback.hpp),
+// written and read with single device-scope atomic accesses: no fence.
+// 64-bit accumulators publish values, wait for them, then one flag per
+// (chunk, tile) (below).  Every wait is bounded (kSpinLimit polls; then the chunk counts as a
 // prefix of the identity and the workspace counts the timeout: a wrong result,
 // never a hang).  The look-back merges in an order that depends on timing, so
 // float sums can differ in the last bit between runs (integers are exact).
