@@ -321,7 +321,7 @@ def test_tiled_scans_one_pass(dtype, kind):
     lib = _abi.lib()
     rng = np.random.default_rng(35)
     for shape, axis, trans in ONE_PASS_CASES:
-        if kind == "cumprod" and (trans or np.prod(shape) > 3_000_000):
+        if kind == "cumprod" and np.prod(shape) > 3_000_000:
             continue
         x = _tiled_input(rng, shape, dtype, kind)
         src = x.T if trans else x
