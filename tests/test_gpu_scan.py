@@ -140,6 +140,42 @@ def test_scan_of_expression_f64():
     assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1)) <= 1e-14
 
 
+def test_scan_of_expression_along_strided_axis_evaluates_first():
+    """cumsum of an expression along a strided axis over 2^20 elements or more
+    evaluates the expression row-major into a temporary of the result type
+    first (accumulator_impl copies before it scans) and scans that with the
+    one-pass tiled kernel, instead of a thread per four columns walking the
+    axis; smaller ones keep the column kernel.  float64 (double literals)
+    against the long-double reference, int64 exact; eager and prepared."""
+    rng = np.random.default_rng(44)
+    a = rng.uniform(-1, 1, (2048, 1024)).astype(np.float32)
+    A_ = Dual.of(a)
+    got = xt.cumsum(A_.dev * 2.0 + 1.0, 0)
+    sync()
+    assert xt.last_kernel() == "scan_tiles_1p"
+    ref = _scan_ref(a.astype(np.float64) * 2.0 + 1.0, 0, "cumsum")
+    _check_scan(got.to_numpy(), ref, "float64", "expr axis 0")
+    b = rng.integers(-100, 100, (3, 700, 520)).astype(np.int32)
+    B = xt.from_numpy(b)
+    got = xt.cumsum(B * 3 - 1, 1)
+    sync()
+    assert xt.last_kernel() == "scan_tiles_1p"
+    ref = np.cumsum(b.astype(np.int64) * 3 - 1, axis=1)
+    assert np.array_equal(got.to_numpy().astype(np.int64), ref)
+    out = xt.empty((2048, 1024), "float64")
+    plan = xt.plan(out, xt.Scan("cumsum", A_.dev * 2.0 + 1.0, 0))
+    for _ in range(2):
+        plan.run()
+        sync()
+        assert xt.last_kernel() == "scan_call"  # the plan keeps its descriptors and runs as xg_scan
+        _check_scan(out.to_numpy(), _scan_ref(a.astype(np.float64) * 2.0 + 1.0, 0, "cumsum"), "float64", "plan")
+    small = rng.uniform(-1, 1, (300, 300)).astype(np.float32)
+    got = xt.cumsum(xt.from_numpy(small) * 2.0, 0)
+    sync()
+    assert xt.last_kernel() == "scan_cols"
+    _check_scan(got.to_numpy(), _scan_ref(small.astype(np.float64) * 2.0, 0, "cumsum"), "float64", "small")
+
+
 @pytest.mark.parametrize("dtype", ["int8", "uint8", "int16", "uint16", "uint32", "uint64", "bfloat16"])
 def test_scans_and_reductions_across_dtypes(dtype):
     """cumsum / cumprod / sum / prod / amax / mean over each axis of a small

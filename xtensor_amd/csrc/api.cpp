@@ -893,6 +893,56 @@ struct ScanPieces {
     }
 };
 
+// A scan along a strided axis that would run the generic column kernel (a
+// thread per V columns walking the axis: an expression, or a leaf of another
+// type or layout) over many elements: evaluate e row-major into a contiguous
+// temporary of the result type first, as accumulator_impl copies before it
+// scans (reducers/xaccumulator.hpp:240-297), and scan the temporary with the
+// tiled kernels.  False (nothing launched) when that does not apply; with
+// `launch` false, only whether it applies (planning, no device work).
+constexpr int64_t kScanEvalFirstMin = int64_t(1) << 20;  // elements
+bool scan_eval_first(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, const ScanPlan& p,
+                     hipStream_t st, bool launch = true) {
+    if (p.layout != SL_COLS || p.tiled != ST_NONE || p.lookback) return false;
+    ExprInfo info = analyse(e);
+    const int64_t n = numel(info.shape);
+    if (n < kScanEvalFirstMin) return false;
+    xg_tensor tmp{};
+    tmp.dtype = p.rdt;
+    tmp.ndim = int32_t(info.shape.size());
+    int64_t stv = 1;
+    for (int k = tmp.ndim - 1; k >= 0; --k) {
+        tmp.shape[k] = info.shape[size_t(k)];
+        tmp.strides[k] = stv;
+        stv *= info.shape[size_t(k)];
+    }
+    tmp.data = reinterpret_cast<void*>(kSizingBase);  // sizing pass: an aligned placeholder
+    xg_node node{};
+    node.op = XG_OP_LEAF;
+    xg_expr te{};
+    te.n_nodes = 1;
+    te.n_leaves = 1;
+    te.nodes = &node;
+    te.leaves = &tmp;
+    const int V = 4, cus = device_cus();
+    ScanPlan q = plan_scan(&te, s, out, V, cus);
+    if (q.tiled == ST_NONE || q.rdt != p.rdt || q.acc != p.acc) return false;
+    if (!launch) return true;
+    const size_t tmp_bytes = (size_t(n) * size_t(dtype_size(p.rdt)) + 255) & ~size_t(255);
+    const size_t wsb = scan_workspace_bytes(q);
+    char* block = static_cast<char*>(workspace(tmp_bytes + wsb, st));
+    tmp.data = block;
+    int Vw = 4;
+    EwPlan ep = plan_assign_best(e, &tmp, &Vw);
+    check_hip(launch_ew(ep, Vw, cus, st, nullptr), "scan copy launch");
+    q = plan_scan(&te, s, out, V, cus);
+    if (scan_workspace_bytes(q) > wsb) raise(XG_INVALID_ARGUMENT, "internal: scan workspace grew after sizing");
+    const char* used = nullptr;
+    check_hip(launch_scan(q, V, wsb ? block + tmp_bytes : nullptr, st, &used), "scan launch");
+    note_kernel(used);
+    return true;
+}
+
 xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, xg_stream_t stream) {
     return guard([&] {
         if (!s || !out) raise(XG_INVALID_ARGUMENT, "null argument");
@@ -914,6 +964,7 @@ xg_status xg_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out,
                 note_kernel(used);
                 return;
             }
+            if (scan_eval_first(e, s, out, p, st)) return;
             check_hip(launch_scan(p, V, workspace(scan_workspace_bytes(p), st), st, &used), "scan launch");
             note_kernel(used);
             return;
@@ -1126,6 +1177,12 @@ xg_status xg_plan_scan(const xg_expr* e, const xg_scan_desc* s, const xg_tensor*
             try {
                 p->scan = plan_scan(e, s, out, p->V, device_cus());
                 p->ws = scan_workspace_bytes(p->scan);
+                if (scan_eval_first(e, s, out, p->scan, nullptr, false)) {
+                    p->kind = 5;  // evaluated first at each execution (xg_scan)
+                    p->keep_call(e, out);
+                    p->d_scan = *s;
+                    p->ws = 0;
+                }
             } catch (const Error& err) {
                 if (err.code != XG_UNSUPPORTED) throw;
                 p->kind = 5;  // past 32-bit indexing: xg_scan scans it in pieces
