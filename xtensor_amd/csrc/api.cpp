@@ -893,17 +893,18 @@ struct ScanPieces {
     }
 };
 
-// A scan along a strided axis that would run the generic column kernel (a
-// thread per V columns walking the axis: an expression, or a leaf of another
-// type or layout) over many elements: evaluate e row-major into a contiguous
-// temporary of the result type first, as accumulator_impl copies before it
-// scans (reducers/xaccumulator.hpp:240-297), and scan the temporary with the
-// tiled kernels.  False (nothing launched) when that does not apply; with
+// A scan that would run the generic kernels (the column kernel, a thread per
+// V columns walking a strided axis, or the rows kernel interpreting the
+// program per element: an expression, or a leaf of another type or layout)
+// over many elements: evaluate e row-major into a contiguous temporary of the
+// result type first, as accumulator_impl copies before it scans
+// (reducers/xaccumulator.hpp:240-297), and scan the temporary with the tiled,
+// direct or long-row kernels.  False (nothing launched) when that does not apply; with
 // `launch` false, only whether it applies (planning, no device work).
 constexpr int64_t kScanEvalFirstMin = int64_t(1) << 20;  // elements
 bool scan_eval_first(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* out, const ScanPlan& p,
                      hipStream_t st, bool launch = true) {
-    if (p.layout != SL_COLS || p.tiled != ST_NONE || p.lookback) return false;
+    if (p.tiled != ST_NONE || p.lookback || (p.layout == SL_ROWS && scan_direct_ok(p))) return false;
     ExprInfo info = analyse(e);
     const int64_t n = numel(info.shape);
     if (n < kScanEvalFirstMin) return false;
@@ -926,7 +927,8 @@ bool scan_eval_first(const xg_expr* e, const xg_scan_desc* s, const xg_tensor* o
     te.leaves = &tmp;
     const int V = 4, cus = device_cus();
     ScanPlan q = plan_scan(&te, s, out, V, cus);
-    if (q.tiled == ST_NONE || q.rdt != p.rdt || q.acc != p.acc) return false;
+    const bool fast = q.tiled != ST_NONE || q.lookback || (q.layout == SL_ROWS && scan_direct_ok(q));
+    if (!fast || q.rdt != p.rdt || q.acc != p.acc) return false;
     if (!launch) return true;
     const size_t tmp_bytes = (size_t(n) * size_t(dtype_size(p.rdt)) + 255) & ~size_t(255);
     const size_t wsb = scan_workspace_bytes(q);
