@@ -141,11 +141,11 @@ def test_scan_of_expression_f64():
 
 
 def test_scan_of_expression_along_strided_axis_evaluates_first():
-    """cumsum of an expression along a strided axis over 2^20 elements or more
-    evaluates the expression row-major into a temporary of the result type
-    first (accumulator_impl copies before it scans) and scans that with the
-    one-pass tiled kernel, instead of a thread per four columns walking the
-    axis; smaller ones keep the column kernel.  float64 (double literals)
+    """cumsum of an expression over 2^20 elements or more evaluates the
+    expression row-major into a temporary of the result type first
+    (accumulator_impl copies before it scans) and scans that with the one-pass
+    tiled kernel (strided axis) or the direct rows kernel (contiguous axis),
+    instead of the generic kernels; smaller ones keep the column kernel.  float64 (double literals)
     against the long-double reference, int64 exact; eager and prepared."""
     rng = np.random.default_rng(44)
     a = rng.uniform(-1, 1, (2048, 1024)).astype(np.float32)
@@ -169,6 +169,10 @@ def test_scan_of_expression_along_strided_axis_evaluates_first():
         sync()
         assert xt.last_kernel() == "scan_call"  # the plan keeps its descriptors and runs as xg_scan
         _check_scan(out.to_numpy(), _scan_ref(a.astype(np.float64) * 2.0 + 1.0, 0, "cumsum"), "float64", "plan")
+    got = xt.cumsum(A_.dev * 2.0 + 1.0, 1)  # the contiguous axis: the direct rows kernel after the copy
+    sync()
+    assert xt.last_kernel().startswith("scan_rows_direct")
+    _check_scan(got.to_numpy(), _scan_ref(a.astype(np.float64) * 2.0 + 1.0, 1, "cumsum"), "float64", "expr axis 1")
     small = rng.uniform(-1, 1, (300, 300)).astype(np.float32)
     got = xt.cumsum(xt.from_numpy(small) * 2.0, 0)
     sync()
